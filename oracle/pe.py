@@ -1,0 +1,525 @@
+"""Primitive equations on sigma levels (oracle restatement).
+
+Restates dinosaur/primitive_equations.py:
+  State                                   :61-84
+  compute_diagnostic_state_sigma          :168-225
+  get_sigma_ratios / geopotential weights :283-336
+  get_temperature_implicit_weights_sigma  :418-487
+  _get_implicit_term_matrix_sigma         :544-598
+  div_sec_lat                             :601-623
+  truncated_modal_orography               :626-636
+  PrimitiveEquationsBase helpers          :670-832 (kinetic energy, orography,
+                                            horizontal scalar advection,
+                                            virtual temperature adjustment)
+  PrimitiveEquationsSigma                 :835-1458 (humidity terms,
+                                            _t_omega_over_sigma_sp,
+                                            curl_and_div_tendencies,
+                                            temperature vertical/adiabatic
+                                            tendencies, log-pressure tendency,
+                                            explicit_terms, implicit_terms,
+                                            implicit_inverse 'split')
+and dinosaur/held_suarez.py:47-167 (HeldSuarezForcingSigma).
+
+Pinned in tests/test_oracle_pe.py by: the implicit-inverse identity
+(I - eta*G)^{-1}(I - eta*G)x = x; entrywise G/H/alpha formulas against
+hand-derived values; the Jablonowski-Williamson steady state being
+stationary under explicit + implicit tendencies (a dropped or mis-signed
+term shows up as an O(1) tendency); zero tendencies for an isothermal rest
+atmosphere on a flat planet; moist equations with q = 0 reducing to dry; and
+the Held-Suarez closed forms.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Mapping
+
+import numpy as np
+
+from . import sht as sht_lib
+from . import sigma as sigma_lib
+
+
+@dataclasses.dataclass
+class State:
+  vorticity: np.ndarray
+  divergence: np.ndarray
+  temperature_variation: np.ndarray
+  log_surface_pressure: np.ndarray
+  tracers: Mapping[str, np.ndarray] = dataclasses.field(default_factory=dict)
+  sim_time: float | None = None
+
+  FIELDS = ('vorticity', 'divergence', 'temperature_variation',
+            'log_surface_pressure')
+
+  def map(self, fn, *others):
+    kw = {f: fn(getattr(self, f), *[getattr(o, f) for o in others])
+          for f in self.FIELDS}
+    kw['tracers'] = {k: fn(v, *[o.tracers[k] for o in others])
+                     for k, v in self.tracers.items()}
+    st = self.sim_time
+    if st is not None and all(o.sim_time is not None for o in others):
+      st = fn(st, *[o.sim_time for o in others])
+    elif others:
+      st = None if any(o.sim_time is None for o in others) else st
+    kw['sim_time'] = st
+    return State(**kw)
+
+  def __add__(self, o):
+    return self.map(lambda a, b: a + b, o)
+
+  def __sub__(self, o):
+    return self.map(lambda a, b: a - b, o)
+
+  def __mul__(self, c):
+    return self.map(lambda a: a * c)
+
+  __rmul__ = __mul__
+
+  def arrays(self):
+    out = {f: getattr(self, f) for f in self.FIELDS}
+    out.update({'tracers/' + k: v for k, v in self.tracers.items()})
+    return out
+
+
+@dataclasses.dataclass
+class Coords:
+  horizontal: sht_lib.Grid
+  vertical: sigma_lib.SigmaCoordinates
+
+  @property
+  def modal_shape(self):
+    return (self.vertical.layers,) + self.horizontal.modal_shape
+
+  @property
+  def nodal_shape(self):
+    return (self.vertical.layers,) + self.horizontal.nodal_shape
+
+  @property
+  def surface_modal_shape(self):
+    return (1,) + self.horizontal.modal_shape
+
+  @property
+  def surface_nodal_shape(self):
+    return (1,) + self.horizontal.nodal_shape
+
+
+# ---------------------------------------------------------------- helpers
+
+def get_sigma_ratios(coords: sigma_lib.SigmaCoordinates) -> np.ndarray:
+  """alpha[j] = log(s[j+1]/s[j])/2, alpha[-1] = -log(s[-1]) (:283-307)."""
+  alpha = np.diff(np.log(coords.centers), append=0) / 2
+  alpha[-1] = -np.log(coords.centers[-1])
+  return alpha
+
+
+def get_geopotential_weights_sigma(coords, ideal_gas_constant):
+  """Upper-triangular G (:310-336)."""
+  alpha = get_sigma_ratios(coords)
+  k = coords.layers
+  w = np.zeros([k, k])
+  for j in range(k):
+    w[j, j] = alpha[j]
+    for kk in range(j + 1, k):
+      w[j, kk] = alpha[kk] + alpha[kk - 1]
+  return ideal_gas_constant * w
+
+
+def get_temperature_implicit_weights_sigma(coords, reference_temperature, kappa):
+  """H (:418-487)."""
+  ref = np.asarray(reference_temperature)
+  if ref.ndim != 1 or ref.shape[-1] != coords.layers:
+    raise ValueError('`reference_temp` must be a vector of length layers')
+  k = coords.layers
+  p = np.tril(np.ones([k, k]))
+  alpha = get_sigma_ratios(coords)[:, None]
+  p_alpha = p * alpha
+  p_alpha_shifted = np.roll(p_alpha, 1, axis=0)
+  p_alpha_shifted[0] = 0
+  dsig = coords.layer_thickness
+  h0 = kappa * ref[:, None] * (p_alpha + p_alpha_shifted) / dsig[:, None]
+  temp_diff = np.diff(ref)
+  thickness_sum = dsig[:-1] + dsig[1:]
+  k0 = np.concatenate((temp_diff / thickness_sum, [0]), axis=0)[:, None]
+  k1 = p - np.cumsum(dsig)[:, None]
+  kk = k0 * k1
+  k_shifted = np.roll(kk, 1, axis=0)
+  k_shifted[0] = 0
+  return (h0 - kk - k_shifted) * dsig
+
+
+def vertical_matvec(a, x):
+  """'gh,...hml->...gml' (:123-125)."""
+  return np.einsum('gh,...hml->...gml', a.astype(x.dtype), x)
+
+
+def vertical_matvec_per_wavenumber(a, x):
+  """'lgh,...hml->...gml' (:128-130)."""
+  return np.einsum('lgh,...hml->...gml', a.astype(x.dtype), x)
+
+
+def implicit_term_matrix(eta, coords: Coords, reference_temperature, kappa,
+                         ideal_gas_constant):
+  """(L, 2K+1, 2K+1) block matrix of (I - eta G) (:544-598)."""
+  k = coords.vertical.layers
+  lam = coords.horizontal.laplacian_eigenvalues
+  g = get_geopotential_weights_sigma(coords.vertical, ideal_gas_constant)
+  h = get_temperature_implicit_weights_sigma(
+      coords.vertical, reference_temperature, kappa)
+  t = np.asarray(reference_temperature, np.float64)[:, None]
+  thickness = coords.vertical.layer_thickness.astype(np.float64)[None, None, :]
+  l = coords.horizontal.modal_shape[1]
+  eye = np.eye(k)[None]
+  row0 = np.concatenate([
+      np.broadcast_to(eye, [l, k, k]),
+      eta * np.einsum('l,jk->ljk', lam, g),
+      eta * ideal_gas_constant * np.einsum('l,jo->ljo', lam, t),
+  ], axis=2)
+  row1 = np.concatenate([
+      eta * np.broadcast_to(h[None], [l, k, k]),
+      np.broadcast_to(eye, [l, k, k]),
+      np.zeros([l, k, 1]),
+  ], axis=2)
+  row2 = np.concatenate([
+      np.broadcast_to(eta * thickness, [l, 1, k]),
+      np.zeros([l, 1, k]),
+      np.ones([l, 1, 1]),
+  ], axis=2)
+  return np.concatenate((row0, row1, row2), axis=1)
+
+
+def div_sec_lat(m_component, n_component, grid):
+  """H(M, N) without the 1/a factor (:601-623)."""
+  mm = grid.to_modal(m_component * grid.sec2_lat)
+  nn = grid.to_modal(n_component * grid.sec2_lat)
+  return grid.div_cos_lat((mm, nn), clip=False)
+
+
+def truncated_modal_orography(orography, coords: Coords, wavenumbers_to_clip=1):
+  """:626-636."""
+  grid = coords.horizontal
+  if orography.shape != grid.nodal_shape:
+    raise ValueError(f'Expected nodal orography with shape={grid.nodal_shape}')
+  return grid.clip_wavenumbers(grid.to_modal(orography), n=wavenumbers_to_clip)
+
+
+@dataclasses.dataclass
+class Diagnostic:
+  vorticity: np.ndarray
+  divergence: np.ndarray
+  temperature_variation: np.ndarray
+  cos_lat_u: tuple
+  sigma_dot_explicit: np.ndarray
+  sigma_dot_full: np.ndarray
+  cos_lat_grad_log_sp: tuple
+  u_dot_grad_log_sp: np.ndarray
+  tracers: dict
+
+
+def compute_diagnostic_state_sigma(state: State, coords: Coords) -> Diagnostic:
+  """:168-225."""
+  grid = coords.horizontal
+  nv = grid.to_nodal(state.vorticity)
+  nd = grid.to_nodal(state.divergence)
+  nt = grid.to_nodal(state.temperature_variation)
+  tracers = {k: grid.to_nodal(v) for k, v in state.tracers.items()}
+  cos_lat_u = tuple(grid.to_nodal(c) for c in sht_lib.get_cos_lat_vector(
+      state.vorticity, state.divergence, grid, clip=False))
+  glsp = grid.cos_lat_grad(state.log_surface_pressure, clip=False)
+  nglsp = tuple(grid.to_nodal(c) for c in glsp)
+  sec2 = grid.sec2_lat.astype(nv.dtype)
+  udg = cos_lat_u[0] * nglsp[0] * sec2 + cos_lat_u[1] * nglsp[1] * sec2
+  f_expl = sigma_lib.cumulative_sigma_integral(udg, coords.vertical)
+  f_full = sigma_lib.cumulative_sigma_integral(nd + udg, coords.vertical)
+  sum_sigma = np.cumsum(coords.vertical.layer_thickness)[:, None, None].astype(nv.dtype)
+  sde = (sum_sigma * f_expl[-1:] - f_expl)[:-1]
+  sdf = (sum_sigma * f_full[-1:] - f_full)[:-1]
+  return Diagnostic(nv, nd, nt, cos_lat_u, sde, sdf, nglsp, udg, tracers)
+
+
+# ---------------------------------------------------------------- equations
+
+@dataclasses.dataclass
+class PrimitiveEquationsSigma:
+  """:835-1458, restated (vertical_matmul_method='dense', 'split' inverse)."""
+  reference_temperature: np.ndarray
+  orography: np.ndarray
+  coords: Coords
+  physics_specs: object
+  include_vertical_advection: bool = True
+  humidity_key: str | None = None
+  cloud_keys: tuple | None = None
+
+  def __post_init__(self):
+    if not np.allclose(self.coords.horizontal.radius,
+                       self.physics_specs.radius, rtol=1e-5):
+      raise ValueError('inconsistent radius between coordinates and constants')
+    if self.cloud_keys is not None and self.humidity_key is None:
+      raise ValueError('cloud_keys requires humidity_key to be set.')
+
+  @property
+  def T_ref(self):
+    return np.asarray(self.reference_temperature)[:, None, None]
+
+  @property
+  def coriolis_parameter(self):
+    _, sin_lat = self.coords.horizontal.nodal_mesh
+    return 2 * self.physics_specs.angular_velocity * sin_lat
+
+  def _vadv(self, w, x):
+    return sigma_lib.centered_vertical_advection(w, x, self.coords.vertical)
+
+  def _q(self, s):
+    return s.tracers[self.humidity_key]
+
+  def _virtual_temperature_adjustment(self, aux):
+    """:749-759."""
+    if self.humidity_key is None:
+      return 1.0
+    ps = self.physics_specs
+    adj = 1 + (ps.R_vapor / ps.R - 1) * self._q(aux)
+    if self.cloud_keys is not None:
+      for key in self.cloud_keys:
+        adj = adj - aux.tracers[key]
+    return adj
+
+  def kinetic_energy_tendency(self, aux):
+    """:778-785."""
+    grid = self.coords.horizontal
+    ke = (aux.cos_lat_u[0] ** 2 + aux.cos_lat_u[1] ** 2) * grid.sec2_lat / 2
+    return -grid.laplacian(grid.to_modal(ke.astype(aux.vorticity.dtype)))
+
+  def orography_tendency(self):
+    """:787-793."""
+    return -self.physics_specs.g * self.coords.horizontal.laplacian(self.orography)
+
+  def horizontal_scalar_advection(self, scalar, aux):
+    """:795-805."""
+    u, v = aux.cos_lat_u
+    nodal = scalar * aux.divergence
+    modal = -div_sec_lat(u * scalar, v * scalar, self.coords.horizontal)
+    return nodal, modal
+
+  def divergence_tendency_due_to_humidity(self, state, aux):
+    """:861-940."""
+    grid = self.coords.horizontal
+    ps = self.physics_specs
+    q = self._q(aux)
+    lap_lsp = grid.to_nodal(grid.laplacian(state.log_surface_pressure))
+    corr = q * lap_lsp * self.T_ref * (ps.R_vapor - ps.R)
+    gq = tuple(grid.to_nodal(c) for c in grid.cos_lat_grad(self._q(state), clip=False))
+    coef = self.T_ref * (ps.R_vapor - ps.R)
+    dot = coef * grid.sec2_lat * (gq[0] * aux.cos_lat_grad_log_sp[0]
+                                  + gq[1] * aux.cos_lat_grad_log_sp[1])
+    temperature = aux.temperature_variation + self.T_ref
+    tdiff = q * temperature * (ps.R_vapor / ps.R - 1)
+    gw = get_geopotential_weights_sigma(self.coords.vertical, ps.R)
+    geo = vertical_matvec(gw, tdiff)
+    return (-grid.laplacian(grid.to_modal(geo))
+            - grid.to_modal(dot + corr))
+
+  def vorticity_tendency_due_to_humidity(self, state, aux):
+    """:942-963."""
+    grid = self.coords.horizontal
+    ps = self.physics_specs
+    gq = tuple(grid.to_nodal(c) for c in grid.cos_lat_grad(self._q(state), clip=False))
+    g = aux.cos_lat_grad_log_sp
+    coef = self.T_ref * (ps.R_vapor - ps.R)
+    curl = coef * grid.sec2_lat * (g[0] * gq[1] - g[1] * gq[0])
+    return grid.to_modal(curl)
+
+  def _t_omega_over_sigma_sp(self, tfield, g_term, v_dot_grad_log_sp):
+    """:965-999."""
+    f = sigma_lib.cumulative_sigma_integral(g_term, self.coords.vertical)
+    alpha = get_sigma_ratios(self.coords.vertical)[:, None, None]
+    dsig = self.coords.vertical.layer_thickness[:, None, None]
+    af = alpha * f
+    af_shift = np.concatenate([np.zeros_like(af[:1]), af[:-1]], axis=0)
+    g_part = (af + af_shift) / dsig
+    return tfield * (v_dot_grad_log_sp - g_part)
+
+  def curl_and_div_tendencies(self, aux):
+    """:1001-1059."""
+    grid = self.coords.horizontal
+    sec2 = grid.sec2_lat
+    u, v = aux.cos_lat_u
+    total_vort = aux.vorticity + self.coriolis_parameter
+    nvu = -v * total_vort * sec2
+    nvv = u * total_vort * sec2
+    sdf = aux.sigma_dot_full
+    if self.include_vertical_advection:
+      sdu = -self._vadv(sdf, u)
+      sdv = -self._vadv(sdf, v)
+    else:
+      sdu = sdv = 0
+    adj = self._virtual_temperature_adjustment(aux)
+    rt = self.physics_specs.R * aux.temperature_variation * adj
+    gu, gv = aux.cos_lat_grad_log_sp
+    vtu = (sdu + rt * gu) * sec2
+    vtv = (sdv + rt * gv) * sec2
+    cu = grid.to_modal(nvu + vtu)
+    cv = grid.to_modal(nvv + vtv)
+    dzeta = -grid.curl_cos_lat((cu, cv), clip=False)
+    ddelta = -grid.div_cos_lat((cu, cv), clip=False)
+    return dzeta, ddelta
+
+  def nodal_temperature_vertical_tendency(self, aux):
+    """:1061-1079."""
+    if self.include_vertical_advection:
+      tend = self._vadv(aux.sigma_dot_full, aux.temperature_variation)
+    else:
+      tend = 0
+    if np.unique(self.T_ref.ravel()).size > 1:
+      tend = tend + self._vadv(aux.sigma_dot_explicit, self.T_ref)
+    return tend
+
+  def nodal_temperature_adiabatic_tendency(self, aux):
+    """:1081-1115."""
+    ps = self.physics_specs
+    g_explicit = aux.u_dot_grad_log_sp
+    g_full = g_explicit + aux.divergence
+    mean_t = self._t_omega_over_sigma_sp(self.T_ref, g_explicit, aux.u_dot_grad_log_sp)
+    if self.humidity_key is None:
+      var_t = self._t_omega_over_sigma_sp(aux.temperature_variation, g_full,
+                                          aux.u_dot_grad_log_sp)
+      return ps.kappa * (mean_t + var_t)
+    gcr = ps.R_vapor / ps.R
+    hcr = ps.Cp_vapor / ps.Cp
+    q = self._q(aux)
+    vtc = aux.temperature_variation * ((1 + (gcr - 1) * q) / (1 + (hcr - 1) * q))
+    hrc = self.T_ref * (((gcr - hcr) * q) / (1 + (hcr - 1) * q))
+    part = self._t_omega_over_sigma_sp(vtc + hrc, g_full, aux.u_dot_grad_log_sp)
+    return ps.kappa * (mean_t + part)
+
+  def nodal_log_pressure_tendency(self, aux):
+    """:1117-1124."""
+    return -sigma_lib.sigma_integral(aux.u_dot_grad_log_sp, self.coords.vertical)
+
+  def explicit_terms(self, state: State) -> State:
+    """:1126-1194."""
+    grid = self.coords.horizontal
+    aux = compute_diagnostic_state_sigma(state, self.coords)
+    vort_dot, div_dot = self.curl_and_div_tendencies(aux)
+    ke = self.kinetic_energy_tendency(aux)
+    oro = self.orography_tendency()
+    if self.humidity_key is not None:
+      vort_dot = vort_dot + self.vorticity_tendency_due_to_humidity(state, aux)
+      div_dot = div_dot + self.divergence_tendency_due_to_humidity(state, aux)
+    dT_hn, dT_hm = self.horizontal_scalar_advection(aux.temperature_variation, aux)
+    tr_h = {k: self.horizontal_scalar_advection(v, aux) for k, v in aux.tracers.items()}
+    dT_v = self.nodal_temperature_vertical_tendency(aux)
+    dT_a = self.nodal_temperature_adiabatic_tendency(aux)
+    lsp = self.nodal_log_pressure_tendency(aux)
+    if self.include_vertical_advection:
+      tr_v = {k: self._vadv(aux.sigma_dot_full, v) for k, v in aux.tracers.items()}
+    else:
+      tr_v = {k: 0 for k in aux.tracers}
+    dtype = state.vorticity.dtype
+    temperature = grid.to_modal(np.asarray(dT_hn + dT_v + dT_a, dtype)) + dT_hm
+    tracers = {k: grid.to_modal(np.asarray(tr_v[k] + tr_h[k][0], dtype)) + tr_h[k][1]
+               for k in aux.tracers}
+    tend = State(
+        vorticity=vort_dot,
+        divergence=div_dot + ke + oro,
+        temperature_variation=temperature,
+        log_surface_pressure=grid.to_modal(np.asarray(lsp, dtype)),
+        tracers=tracers,
+        sim_time=None if state.sim_time is None else 1.0,
+    )
+    return tend.map(lambda x: grid.clip_wavenumbers(x) if isinstance(x, np.ndarray) else x)
+
+  def implicit_terms(self, state: State) -> State:
+    """:1196-1249 (dense vertical matvecs)."""
+    grid = self.coords.horizontal
+    ps = self.physics_specs
+    gw = get_geopotential_weights_sigma(self.coords.vertical, ps.R)
+    geo = vertical_matvec(gw, state.temperature_variation)
+    rt_log_p = ps.ideal_gas_constant * self.T_ref * state.log_surface_pressure
+    div = -grid.laplacian(geo + rt_log_p)
+    hw = -get_temperature_implicit_weights_sigma(
+        self.coords.vertical, self.reference_temperature, ps.kappa)
+    temp = vertical_matvec(hw, state.divergence)
+    lsp = -vertical_matvec(self.coords.vertical.layer_thickness[None], state.divergence)
+    return State(
+        vorticity=np.zeros_like(state.vorticity),
+        divergence=div.astype(state.divergence.dtype),
+        temperature_variation=temp,
+        log_surface_pressure=lsp,
+        tracers={k: np.zeros_like(v) for k, v in state.tracers.items()},
+        sim_time=None if state.sim_time is None else 0.0,
+    )
+
+  def implicit_inverse(self, state: State, step_size: float) -> State:
+    """:1251-1458, 'split' method."""
+    if not isinstance(step_size, (float, int, np.floating)):
+      raise TypeError('`step_size` must be concrete')
+    ps = self.physics_specs
+    mat = implicit_term_matrix(step_size, self.coords, self.reference_temperature,
+                               ps.kappa, ps.R)
+    inv = np.linalg.inv(mat)
+    assert not np.isnan(inv).any()
+    k = self.coords.vertical.layers
+    div, temp, logp = slice(0, k), slice(k, 2 * k), slice(2 * k, 2 * k + 1)
+    vm = vertical_matvec_per_wavenumber
+    d, t, lp = state.divergence, state.temperature_variation, state.log_surface_pressure
+    new_d = vm(inv[:, div, div], d) + vm(inv[:, div, temp], t) + vm(inv[:, div, logp], lp)
+    new_t = vm(inv[:, temp, div], d) + vm(inv[:, temp, temp], t) + vm(inv[:, temp, logp], lp)
+    new_lp = vm(inv[:, logp, div], d) + vm(inv[:, logp, temp], t) + vm(inv[:, logp, logp], lp)
+    return State(state.vorticity, new_d, new_t, new_lp, dict(state.tracers),
+                 state.sim_time)
+
+
+class HeldSuarezForcingSigma:
+  """held_suarez.py:47-167."""
+
+  def __init__(self, coords: Coords, physics_specs, reference_temperature,
+               p0=None, sigma_b=0.7, kf=None, ka=None, ks=None, minT=None,
+               maxT=None, dTy=None, dThz=None):
+    from . import units as u
+    self.coords = coords
+    self.physics_specs = physics_specs
+    self.reference_temperature = np.asarray(reference_temperature)
+    self.p0 = u.pascal(1e5) if p0 is None else p0
+    self.sigma_b = sigma_b
+    self.kf = u.per_day(1) if kf is None else kf
+    self.ka = u.per_day(40) if ka is None else ka
+    self.ks = u.per_day(4) if ks is None else ks
+    self.minT = 200.0 if minT is None else minT
+    self.maxT = 315.0 if maxT is None else maxT
+    self.dTy = 60.0 if dTy is None else dTy
+    self.dThz = 10.0 if dThz is None else dThz
+    self.sigma = coords.vertical.centers.astype(np.float64)
+    _, sin_lat = coords.horizontal.nodal_mesh
+    self.lat = np.arcsin(sin_lat)
+
+  def kv(self):
+    c = self.kf * np.maximum(0, (self.sigma - self.sigma_b) / (1 - self.sigma_b))
+    return c[:, None, None]
+
+  def kt(self):
+    cut = np.maximum(0, (self.sigma - self.sigma_b) / (1 - self.sigma_b))
+    return self.ka + (self.ks - self.ka) * (cut[:, None, None] * np.cos(self.lat) ** 4)
+
+  def equilibrium_temperature(self, nodal_surface_pressure):
+    p_over_p0 = self.sigma[:, None, None] * nodal_surface_pressure / self.p0
+    t = p_over_p0 ** self.physics_specs.kappa * (
+        self.maxT - self.dTy * np.sin(self.lat) ** 2
+        - self.dThz * np.log(p_over_p0) * np.cos(self.lat) ** 2)
+    return np.maximum(self.minT, t)
+
+  def explicit_terms(self, state: State) -> State:
+    grid = self.coords.horizontal
+    dtype = state.vorticity.dtype
+    aux = compute_diagnostic_state_sigma(state, self.coords)
+    nvt = tuple((-self.kv() * x / grid.cos_lat ** 2).astype(dtype) for x in aux.cos_lat_u)
+    temp = self.reference_temperature[:, None, None] + aux.temperature_variation
+    lsp = grid.to_nodal(state.log_surface_pressure)
+    teq = self.equilibrium_temperature(np.exp(lsp))
+    ttend = (-self.kt() * (temp - teq)).astype(dtype)
+    t_modal = grid.to_modal(ttend)
+    vt = grid.to_modal(nvt)
+    return State(
+        vorticity=grid.curl_cos_lat(vt),
+        divergence=grid.div_cos_lat(vt),
+        temperature_variation=t_modal,
+        log_surface_pressure=np.zeros_like(state.log_surface_pressure),
+    )

@@ -1,0 +1,169 @@
+"""IMEX time stepping and filters (oracle restatement).
+
+Restates dinosaur/time_integration.py:
+  compose_equations              :141-161
+  ImExButcherTableau / imex_runge_kutta / imex_rk_sil3   :313-405
+  exponential_step_filter        :427-455
+  step_with_filters / repeated / trajectory_from_step    :494-564
+and dinosaur/filtering.py:38-89 (exponential_filter, shape-gated scaling).
+
+Pinned in tests/test_oracle_time.py by closed-form ODE solutions (the SIL3
+third-order explicit / second-order implicit convergence on a split linear
+ODE) and the filter KAT exp(-16) at the top wavenumber.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from .pe import State
+
+
+@dataclasses.dataclass
+class Tableau:
+  a_ex: list
+  a_im: list
+  b_ex: list
+  b_im: list
+
+
+SIL3 = Tableau(
+    a_ex=[[1 / 3], [1 / 6, 1 / 2], [1 / 2, -1 / 2, 1]],
+    a_im=[[1 / 6, 1 / 6], [1 / 3, 0, 1 / 3], [3 / 8, 0, 3 / 8, 1 / 4]],
+    b_ex=[1 / 2, -1 / 2, 1, 0],
+    b_im=[3 / 8, 0, 3 / 8, 1 / 4],
+)
+
+
+def _lin(terms):
+  """sum(c * x) over (c, x) pairs for State or ndarray."""
+  out = None
+  for c, x in terms:
+    t = x * c
+    out = t if out is None else out + t
+  return out
+
+
+def imex_runge_kutta(tableau, equation, time_step):
+  """time_integration.py:333-370."""
+  dt = time_step
+  F, G, G_inv = equation.explicit_terms, equation.implicit_terms, equation.implicit_inverse
+  a_ex, a_im, b_ex, b_im = tableau.a_ex, tableau.a_im, tableau.b_ex, tableau.b_im
+  n = len(b_ex)
+
+  def step_fn(y0):
+    f = [None] * n
+    g = [None] * n
+    f[0] = F(y0)
+    g[0] = G(y0)
+    for i in range(1, n):
+      ex = [(dt * a_ex[i - 1][j], f[j]) for j in range(i) if a_ex[i - 1][j]]
+      im = [(dt * a_im[i - 1][j], g[j]) for j in range(i) if a_im[i - 1][j]]
+      y_star = _lin([(1.0, y0)] + ex + im)
+      y = G_inv(y_star, dt * a_im[i - 1][i])
+      if any(a_ex[j][i] for j in range(i, n - 1)) or b_ex[i]:
+        f[i] = F(y)
+      if any(a_im[j][i] for j in range(i, n - 1)) or b_im[i]:
+        g[i] = G(y)
+    ex = [(dt * b_ex[j], f[j]) for j in range(n) if b_ex[j]]
+    im = [(dt * b_im[j], g[j]) for j in range(n) if b_im[j]]
+    return _lin([(1.0, y0)] + ex + im)
+
+  return step_fn
+
+
+def imex_rk_sil3(equation, time_step):
+  return imex_runge_kutta(SIL3, equation, time_step)
+
+
+class Composed:
+  """compose_equations (time_integration.py:141-161)."""
+
+  def __init__(self, equations):
+    imex = [e for e in equations if hasattr(e, 'implicit_inverse')]
+    if len(imex) != 1:
+      raise ValueError('compose_equations supports at most 1 ImplicitExplicitODE')
+    self.equations = equations
+    self.imex = imex[0]
+
+  def explicit_terms(self, x):
+    terms = [e.explicit_terms(x) for e in self.equations]
+    out = terms[0]
+    for t in terms[1:]:
+      out = State(
+          out.vorticity + t.vorticity, out.divergence + t.divergence,
+          out.temperature_variation + t.temperature_variation,
+          out.log_surface_pressure + t.log_surface_pressure,
+          {k: v + t.tracers.get(k, 0) for k, v in out.tracers.items()},
+          out.sim_time if out.sim_time is not None else t.sim_time)
+    return out
+
+  def implicit_terms(self, x):
+    return self.imex.implicit_terms(x)
+
+  def implicit_inverse(self, x, eta):
+    return self.imex.implicit_inverse(x, eta)
+
+
+def exponential_filter_scaling(grid, attenuation=16, order=18, cutoff=0):
+  """filtering.py:38-89: scaling over total wavenumber."""
+  _, l = grid.modal_axes
+  k = l / l.max()
+  return np.exp((k > cutoff) * (-attenuation * (((k - cutoff) / (1 - cutoff)) ** (2 * order))))
+
+
+def exponential_filter(grid, attenuation=16, order=18, cutoff=0):
+  scaling = exponential_filter_scaling(grid, attenuation, order, cutoff)
+
+  def rescale(x):
+    if x is None:
+      return x
+    shp = np.shape(x)
+    if shp == np.broadcast_shapes(shp, scaling.shape):
+      return (x * scaling).astype(x.dtype)
+    return x
+
+  def apply(state):
+    if isinstance(state, State):
+      return state.map(rescale)
+    return rescale(state)
+
+  return apply
+
+
+def exponential_step_filter(grid, dt, tau=0.010938, order=18, cutoff=0):
+  """time_integration.py:427-455."""
+  f = exponential_filter(grid, dt / tau, order, cutoff)
+  return lambda u, u_next: f(u_next)
+
+
+def step_with_filters(step_fn, filters):
+  def _step(u):
+    u_next = step_fn(u)
+    for flt in filters:
+      u_next = flt(u, u_next)
+    return u_next
+  return _step
+
+
+def repeated(fn, steps):
+  def f(x):
+    for _ in range(steps):
+      x = fn(x)
+    return x
+  return f
+
+
+def trajectory_from_step(step_fn, outer_steps, inner_steps, start_with_input=False):
+  inner = repeated(step_fn, inner_steps)
+
+  def multistep(x):
+    frames = []
+    for _ in range(outer_steps):
+      y = inner(x)
+      frames.append(x if start_with_input else y)
+      x = y
+    return x, frames
+
+  return multistep

@@ -1,0 +1,108 @@
+"""ctypes binding of libdinosaur_amd.so (the gfx950 HIP kernels).
+
+The library is built in-tree by ``python -m dinosaur_amd.build``.  There is
+no fallback: if the library is missing, loading raises ImportError, and every
+compute path of the package goes through it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, '_lib', 'libdinosaur_amd.so')
+
+DINO_OK, DINO_EINVAL, DINO_ESHAPE, DINO_EHIP, DINO_ENOMEM = range(5)
+DINO_F32, DINO_F64 = 0, 1
+PREC_NATIVE, PREC_F16X3, PREC_BF16X3 = 0, 1, 2
+PRECISIONS = {'native': PREC_NATIVE, 'float32': PREC_NATIVE, 'float64': PREC_NATIVE,
+              'f16x3': PREC_F16X3, 'bf16x3': PREC_BF16X3}
+
+_c_p = ctypes.c_void_p
+_c_i = ctypes.c_int
+_c_i64 = ctypes.c_int64
+_c_d = ctypes.c_double
+_c_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); every symbol declared in include/dinosaur_amd.h
+SIGNATURES = {
+    'dino_last_error': (ctypes.c_char_p, []),
+    'dino_version': (ctypes.c_char_p, []),
+    'dino_gauss_nodes_host': (_c_i, [_c_i, _c_dp, _c_dp]),
+    'dino_plan_create': (_c_i, [_c_i, _c_i, _c_i, _c_i, _c_dp, _c_dp, ctypes.POINTER(_c_p)]),
+    'dino_plan_destroy': (_c_i, [_c_p]),
+    'dino_plan_nodes': (_c_i, [_c_p, _c_dp, _c_dp]),
+    'dino_plan_table_size': (_c_i64, [_c_p]),
+    'dino_plan_table': (_c_i, [_c_p, _c_dp]),
+    'dino_sht_workspace_bytes': (ctypes.c_size_t, [_c_p, _c_i64, _c_i]),
+    'dino_sht_inverse': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
+    'dino_sht_forward': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
+    'dino_sht_inverse_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
+    'dino_sht_forward_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
+    'dino_dft_forward': (_c_i, [_c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
+    'dino_dft_inverse': (_c_i, [_c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
+    'dino_legendre_inverse': (_c_i, [_c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
+    'dino_legendre_forward': (_c_i, [_c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
+    'dino_implicit_create': (_c_i, [_c_i, _c_i, _c_i, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp,
+                                    _c_dp, _c_dp, _c_dp, _c_d, _c_d, ctypes.POINTER(_c_p)]),
+    'dino_implicit_destroy': (_c_i, [_c_p]),
+    'dino_implicit_solve': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
+    'dino_implicit_terms': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class StateShapeError(Exception):
+  """Exceptions for unexpected state shapes (primitive_equations.py:87-88)."""
+
+
+def lib() -> ctypes.CDLL:
+  """Loads the HIP library (raises ImportError if it was not built)."""
+  global _lib
+  if _lib is not None:
+    return _lib
+  with _lock:
+    if _lib is None:
+      if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f'{LIB_PATH} is missing: build it with `python -m dinosaur_amd.build` '
+            '(dinosaur_amd has no CPU fallback).')
+      handle = ctypes.CDLL(LIB_PATH)
+      for name, (res, args) in SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+      _lib = handle
+  return _lib
+
+
+def check(rc: int, what: str = '') -> None:
+  if rc == DINO_OK:
+    return
+  msg = lib().dino_last_error().decode()
+  if what:
+    msg = f'{what}: {msg}'
+  if rc == DINO_EINVAL:
+    raise ValueError(msg)
+  if rc == DINO_ESHAPE:
+    raise StateShapeError(msg)
+  if rc == DINO_ENOMEM:
+    raise MemoryError(msg)
+  raise RuntimeError(msg)
+
+
+def dptr(a: np.ndarray):
+  assert a.dtype == np.float64 and a.flags['C_CONTIGUOUS']
+  return a.ctypes.data_as(_c_dp)
+
+
+def gauss_nodes_host(n: int):
+  x = np.empty(n)
+  w = np.empty(n)
+  check(lib().dino_gauss_nodes_host(n, dptr(x), dptr(w)), 'gauss nodes')
+  return x, w

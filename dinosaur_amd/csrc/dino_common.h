@@ -1,0 +1,78 @@
+// Internal definitions shared by the dinosaur_amd HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/dinosaur_amd.h"
+
+namespace dino {
+
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+
+#define DINO_HIP(call)                                   \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return ::dino::hip_fail(e_, #call); \
+  } while (0)
+
+#define DINO_LAUNCH_CHECK(what)                          \
+  do {                                                   \
+    hipError_t e_ = hipGetLastError();                   \
+    if (e_ != hipSuccess) return ::dino::hip_fail(e_, what); \
+  } while (0)
+
+#define DINO_REQUIRE(cond, code, ...)                    \
+  do {                                                   \
+    if (!(cond)) {                                       \
+      ::dino::set_error(__VA_ARGS__);                    \
+      return code;                                       \
+    }                                                    \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace dino
+
+// Transform plan: everything lives on the device and is immutable after
+// creation, so a plan may be shared by threads and streams.
+struct dino_plan {
+  int M, L, I, J;
+  int R;        // 2M - 1 modal rows
+  int Jh;       // ceil(J / 2): rows of the folded (northern) hemisphere
+  int Jhp;      // Jh padded to a multiple of 8 (table row stride)
+  bool pow2;    // I is a power of two -> FFT path
+  int64_t table_elems;  // sum_m (L - m) * Jhp
+  // device arrays
+  double* nodes64;      // (J) ascending
+  double* weights64;    // (J) Gauss weights (sum 2)
+  float* qw32;          // (J) (2 pi / I) * w_j
+  double* qw64;
+  int64_t* m_offset;    // (M) start of row l = m of each m block in the table
+  double* p64;          // folded table, FP64
+  float* p32;           // FP32 copy
+  uint16_t* p16_hi;     // split FP16 hi/lo
+  uint16_t* p16_lo;
+  uint16_t* pbf_hi;     // split BF16 hi/lo
+  uint16_t* pbf_lo;
+  double* tw64;         // (I) complex twiddles exp(-2 pi i k / I) as (cos, sin)
+  float* tw32;
+};
+
+struct dino_implicit {
+  int K, L, R;
+  double r_gas, eta;
+  // device FP64: G, H, V, Vinv (K*K each), tref, dsigma, d (K each), lam (L)
+  double* G;
+  double* H;
+  double* V;
+  double* Vinv;
+  double* tref;
+  double* dsigma;
+  double* d;
+  double* lam;
+};

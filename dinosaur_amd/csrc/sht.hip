@@ -1,0 +1,113 @@
+// C-ABI entry points of the spherical harmonic transform: the longitude DFT
+// (dft.hip) and the folded Legendre GEMMs (legendre*.hip) chained through a
+// caller-provided workspace holding the (batch, 2M-1, J) Fourier rows.
+#include "dino_common.h"
+
+namespace dino {
+int dft_forward_impl(const dino_plan*, const void*, void*, int64_t, int, int, hipStream_t);
+int dft_inverse_impl(const dino_plan*, const void*, void*, int64_t, int, int, hipStream_t);
+int legendre_inverse_fma(const dino_plan*, const void*, void*, int64_t, int, hipStream_t);
+int legendre_forward_fma(const dino_plan*, const void*, void*, int64_t, int, hipStream_t);
+int legendre_inverse_mfma(const dino_plan*, const void*, void*, int64_t, int, hipStream_t);
+int legendre_forward_mfma(const dino_plan*, const void*, void*, int64_t, int, hipStream_t);
+}  // namespace dino
+
+using namespace dino;
+
+namespace {
+
+int check_common(const dino_plan* p, const void* a, const void* b, int64_t batch, int dtype,
+                 int precision) {
+  DINO_REQUIRE(p != nullptr, DINO_EINVAL, "NULL plan");
+  DINO_REQUIRE(batch >= 0, DINO_EINVAL, "negative batch");
+  DINO_REQUIRE(batch == 0 || (a && b), DINO_EINVAL, "NULL buffer");
+  DINO_REQUIRE(dtype == DINO_F32 || dtype == DINO_F64, DINO_EINVAL, "unknown dtype %d", dtype);
+  DINO_REQUIRE(precision == DINO_PREC_NATIVE || precision == DINO_PREC_F16X3 ||
+                   precision == DINO_PREC_BF16X3,
+               DINO_EINVAL, "unknown precision %d", precision);
+  DINO_REQUIRE(precision == DINO_PREC_NATIVE || dtype == DINO_F32, DINO_EINVAL,
+               "split-precision transforms take FP32 data");
+  return DINO_OK;
+}
+
+int leg_inverse(const dino_plan* p, const void* in, void* out, int64_t B, int dtype, int prec,
+                hipStream_t s) {
+  if (prec == DINO_PREC_NATIVE) return legendre_inverse_fma(p, in, out, B, dtype, s);
+  return legendre_inverse_mfma(p, in, out, B, prec, s);
+}
+
+int leg_forward(const dino_plan* p, const void* in, void* out, int64_t B, int dtype, int prec,
+                hipStream_t s) {
+  if (prec == DINO_PREC_NATIVE) return legendre_forward_fma(p, in, out, B, dtype, s);
+  return legendre_forward_mfma(p, in, out, B, prec, s);
+}
+
+}  // namespace
+
+#define CHECK_OR_RETURN(expr)    \
+  do {                           \
+    int rc_ = (expr);            \
+    if (rc_ != DINO_OK) return rc_; \
+  } while (0)
+
+extern "C" int dino_sht_inverse(const dino_plan* p, const void* modal, void* nodal, void* work,
+                                int64_t batch, int dtype, int precision, void* stream) {
+  CHECK_OR_RETURN(check_common(p, modal, nodal, batch, dtype, precision));
+  DINO_REQUIRE(batch == 0 || work, DINO_EINVAL, "NULL workspace");
+  hipStream_t s = as_stream(stream);
+  CHECK_OR_RETURN(leg_inverse(p, modal, work, batch, dtype, precision, s));
+  return dft_inverse_impl(p, work, nodal, batch, dtype, 0, s);
+}
+
+extern "C" int dino_sht_forward(const dino_plan* p, const void* nodal, void* modal, void* work,
+                                int64_t batch, int dtype, int precision, void* stream) {
+  CHECK_OR_RETURN(check_common(p, nodal, modal, batch, dtype, precision));
+  DINO_REQUIRE(batch == 0 || work, DINO_EINVAL, "NULL workspace");
+  hipStream_t s = as_stream(stream);
+  CHECK_OR_RETURN(dft_forward_impl(p, nodal, work, batch, dtype, 1, s));
+  return leg_forward(p, work, modal, batch, dtype, precision, s);
+}
+
+extern "C" int dino_sht_inverse_adjoint(const dino_plan* p, const void* nodal_cot,
+                                        void* modal_cot, void* work, int64_t batch, int dtype,
+                                        int precision, void* stream) {
+  CHECK_OR_RETURN(check_common(p, nodal_cot, modal_cot, batch, dtype, precision));
+  DINO_REQUIRE(batch == 0 || work, DINO_EINVAL, "NULL workspace");
+  hipStream_t s = as_stream(stream);
+  CHECK_OR_RETURN(dft_forward_impl(p, nodal_cot, work, batch, dtype, 0, s));
+  return leg_forward(p, work, modal_cot, batch, dtype, precision, s);
+}
+
+extern "C" int dino_sht_forward_adjoint(const dino_plan* p, const void* modal_cot,
+                                        void* nodal_cot, void* work, int64_t batch, int dtype,
+                                        int precision, void* stream) {
+  CHECK_OR_RETURN(check_common(p, modal_cot, nodal_cot, batch, dtype, precision));
+  DINO_REQUIRE(batch == 0 || work, DINO_EINVAL, "NULL workspace");
+  hipStream_t s = as_stream(stream);
+  CHECK_OR_RETURN(leg_inverse(p, modal_cot, work, batch, dtype, precision, s));
+  return dft_inverse_impl(p, work, nodal_cot, batch, dtype, 1, s);
+}
+
+extern "C" int dino_dft_forward(const dino_plan* p, const void* nodal, void* fourier,
+                                int64_t batch, int dtype, int weighted, void* stream) {
+  CHECK_OR_RETURN(check_common(p, nodal, fourier, batch, dtype, DINO_PREC_NATIVE));
+  return dft_forward_impl(p, nodal, fourier, batch, dtype, weighted, as_stream(stream));
+}
+
+extern "C" int dino_dft_inverse(const dino_plan* p, const void* fourier, void* nodal,
+                                int64_t batch, int dtype, int weighted, void* stream) {
+  CHECK_OR_RETURN(check_common(p, fourier, nodal, batch, dtype, DINO_PREC_NATIVE));
+  return dft_inverse_impl(p, fourier, nodal, batch, dtype, weighted, as_stream(stream));
+}
+
+extern "C" int dino_legendre_inverse(const dino_plan* p, const void* modal, void* fourier,
+                                     int64_t batch, int dtype, int precision, void* stream) {
+  CHECK_OR_RETURN(check_common(p, modal, fourier, batch, dtype, precision));
+  return leg_inverse(p, modal, fourier, batch, dtype, precision, as_stream(stream));
+}
+
+extern "C" int dino_legendre_forward(const dino_plan* p, const void* fourier, void* modal,
+                                     int64_t batch, int dtype, int precision, void* stream) {
+  CHECK_OR_RETURN(check_common(p, fourier, modal, batch, dtype, precision));
+  return leg_forward(p, fourier, modal, batch, dtype, precision, as_stream(stream));
+}

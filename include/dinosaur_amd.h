@@ -1,0 +1,169 @@
+/*
+ * dinosaur_amd C ABI: MI355X (gfx950) kernels for the spectral dynamical core
+ * of neuralgcm/dinosaur.  Plain pointers and sizes only; every device buffer
+ * is allocated by the caller; every compute call is asynchronous on the
+ * caller's HIP stream (passed as void*, 0 = default stream) and does no host
+ * synchronisation and no device allocation.
+ *
+ * Layouts (row-major, last index contiguous), matching the reference:
+ *   modal  (batch, 2M-1, L)   Real layout, m rows ordered [0, +1, -1, +2, ...]
+ *                              (dinosaur/spherical_harmonic.py:192-200)
+ *   nodal  (batch, I, J)      longitude x sin(latitude), latitude contiguous
+ *                              (dinosaur/spherical_harmonic.py:186-188)
+ *   fourier (batch, 2M-1, J)  longitude-transformed intermediate
+ *
+ * Return codes: DINO_OK, or an error whose message dino_last_error() returns
+ * (thread-local).  The Python layer maps DINO_EINVAL -> ValueError,
+ * DINO_ESHAPE -> StateShapeError, others -> RuntimeError.
+ */
+#ifndef DINOSAUR_AMD_H_
+#define DINOSAUR_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DINO_OK = 0,
+  DINO_EINVAL = 1,
+  DINO_ESHAPE = 2,
+  DINO_EHIP = 3,
+  DINO_ENOMEM = 4,
+};
+
+/* element type of caller buffers */
+enum { DINO_F32 = 0, DINO_F64 = 1 };
+
+/* arithmetic of the Legendre contraction (FP32 data only for the split forms) */
+enum {
+  DINO_PREC_NATIVE = 0, /* FP32 FMA for FP32 data, FP64 FMA for FP64 data */
+  DINO_PREC_F16X3 = 1,  /* split-FP16 x3 on MFMA, FP32 accumulation */
+  DINO_PREC_BF16X3 = 2, /* split-BF16 x3 on MFMA, FP32 accumulation */
+};
+
+typedef struct dino_plan dino_plan;
+typedef struct dino_implicit dino_implicit;
+
+/* Last error message of the calling thread ("" if none). */
+const char* dino_last_error(void);
+
+/* Library version string. */
+const char* dino_version(void);
+
+/* FP64 Gauss-Legendre nodes (ascending) and weights on the host, by the same
+ * Newton iteration the plan runs on the GPU.
+ * Replaces associated_legendre.gauss_legendre_nodes
+ * (dinosaur/associated_legendre.py:104-114). */
+int dino_gauss_nodes_host(int n, double* nodes, double* weights);
+
+/* Builds a transform plan: FP64 Gauss nodes/weights (or the given nodes and
+ * weights) and the FP64 normalised P_l^m table generated on the GPU, folded
+ * onto the northern hemisphere, plus FP32 / split-FP16 / split-BF16 copies
+ * and the longitude twiddle table.  nodes/weights: host arrays of length J,
+ * ascending and symmetric about 0, or NULL for Gauss nodes.
+ * Replaces RealSphericalHarmonics.basis
+ * (dinosaur/spherical_harmonic.py:219-260) together with
+ * associated_legendre.evaluate (:70-101) and fourier.real_basis
+ * (dinosaur/fourier.py:24-60). */
+int dino_plan_create(int M, int L, int I, int J, const double* nodes,
+                     const double* weights, dino_plan** out);
+int dino_plan_destroy(dino_plan* plan);
+
+/* Copies the plan's FP64 nodes/weights (length J) to host arrays. */
+int dino_plan_nodes(const dino_plan* plan, double* nodes, double* weights);
+
+/* Copies the folded FP64 P table to a host array: for each m, rows l = m..L-1,
+ * each of Jh = ceil(J/2) values at the nodes j = J-Jh .. J-1 (test hook). */
+int64_t dino_plan_table_size(const dino_plan* plan);
+int dino_plan_table(const dino_plan* plan, double* out);
+
+/* Bytes of workspace the transforms below need for `batch` fields. */
+size_t dino_sht_workspace_bytes(const dino_plan* plan, int64_t batch,
+                                int dtype);
+
+/* modal -> nodal.  Replaces RealSphericalHarmonics.inverse_transform
+ * (dinosaur/spherical_harmonic.py:262-269) mapped over a batch by
+ * Grid.to_nodal (:1017-1023). */
+int dino_sht_inverse(const dino_plan* plan, const void* modal, void* nodal,
+                     void* work, int64_t batch, int dtype, int precision,
+                     void* stream);
+
+/* nodal -> modal with quadrature weights.  Replaces
+ * RealSphericalHarmonics.transform (:271-280) via Grid.to_modal (:1025-1031). */
+int dino_sht_forward(const dino_plan* plan, const void* nodal, void* modal,
+                     void* work, int64_t batch, int dtype, int precision,
+                     void* stream);
+
+/* Adjoints (vector-Jacobian products): inverse_adjoint maps a nodal
+ * cotangent to a modal one (P^T F^T y), forward_adjoint maps a modal
+ * cotangent to a nodal one (w * F P x).  The reference obtains these by XLA
+ * transposition of the einsums at :265-279. */
+int dino_sht_inverse_adjoint(const dino_plan* plan, const void* nodal_cot,
+                             void* modal_cot, void* work, int64_t batch,
+                             int dtype, int precision, void* stream);
+int dino_sht_forward_adjoint(const dino_plan* plan, const void* modal_cot,
+                             void* nodal_cot, void* work, int64_t batch,
+                             int dtype, int precision, void* stream);
+
+/* Individual stages (for tests and roofline measurement).
+ * dft_forward: nodal (B,I,J) -> fourier (B,2M-1,J); weighted != 0 multiplies
+ *   by the quadrature weight (2 pi / I) * w_j (the reference's `w * x`, :275).
+ * dft_inverse: fourier -> nodal; weighted != 0 multiplies the output by the
+ *   same weight (used by the forward adjoint).
+ * legendre_inverse: modal -> fourier; legendre_forward: fourier -> modal. */
+int dino_dft_forward(const dino_plan* plan, const void* nodal, void* fourier,
+                     int64_t batch, int dtype, int weighted, void* stream);
+int dino_dft_inverse(const dino_plan* plan, const void* fourier, void* nodal,
+                     int64_t batch, int dtype, int weighted, void* stream);
+int dino_legendre_inverse(const dino_plan* plan, const void* modal,
+                          void* fourier, int64_t batch, int dtype,
+                          int precision, void* stream);
+int dino_legendre_forward(const dino_plan* plan, const void* fourier,
+                          void* modal, int64_t batch, int dtype,
+                          int precision, void* stream);
+
+/* Semi-implicit solve (1 - eta * implicit_terms)^{-1} for the divergence,
+ * temperature-variation and log-surface-pressure fields, factorised once per
+ * (grid, levels, eta) in FP64 through the vertical modes of the l-independent
+ * matrix A = G H + R T_ref dsigma^T:
+ *   (I - eta^2 lambda_l A)^{-1} = V diag(1 / (1 - eta^2 lambda_l d)) V^{-1}.
+ * All matrices are K x K row-major FP64 host arrays: G (geopotential
+ * weights), H (temperature implicit weights, as returned by
+ * get_temperature_implicit_weights_sigma), V and Vinv (eigenvectors of A and
+ * their inverse), d (eigenvalues of A, length K); tref and dsigma length K;
+ * lam length L (laplacian eigenvalues).
+ * Replaces PrimitiveEquationsSigma.implicit_inverse with the host
+ * np.linalg.inv of _get_implicit_term_matrix_sigma
+ * (dinosaur/primitive_equations.py:1251-1458, :544-598). */
+int dino_implicit_create(int K, int L, int R, const double* G, const double* H,
+                         const double* tref, const double* dsigma,
+                         const double* lam, const double* V,
+                         const double* Vinv, const double* d, double r_gas,
+                         double eta, dino_implicit** out);
+int dino_implicit_destroy(dino_implicit* f);
+
+/* In place? No: reads div/temp (members, K, R, L) and lnps (members, 1, R, L)
+ * and writes the three outputs (which may not alias the inputs). */
+int dino_implicit_solve(const dino_implicit* f, const void* div,
+                        const void* temp, const void* lnps, void* div_out,
+                        void* temp_out, void* lnps_out, int64_t members,
+                        int dtype, void* stream);
+
+/* Implicit tendencies (primitive_equations.py:1196-1249): writes
+ *   div_t  = -lambda_l (G T' + R T_ref lnps)
+ *   temp_t = -H delta
+ *   lnps_t = -dsigma . delta
+ * using the same factor object's G, H, tref, dsigma and lam (eta unused). */
+int dino_implicit_terms(const dino_implicit* f, const void* div,
+                        const void* temp, const void* lnps, void* div_out,
+                        void* temp_out, void* lnps_out, int64_t members,
+                        int dtype, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DINOSAUR_AMD_H_ */

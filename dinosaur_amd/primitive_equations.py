@@ -1,0 +1,701 @@
+"""Primitive equations on sigma levels, for MI355X.
+
+Drop-in for the sigma path of dinosaur/primitive_equations.py: ``State``
+(:61-84), the vertical weight matrices (:283-541), ``div_sec_lat`` (:601-623),
+``truncated_modal_orography`` (:626-636), ``PrimitiveEquationsSigma``
+(:835-1458) with ``explicit_terms`` / ``implicit_terms`` / ``implicit_inverse``
+and the deprecated aliases ``PrimitiveEquations``, ``MoistPrimitiveEquations``
+and ``MoistPrimitiveEquationsWithCloudMoisture`` (:2534-2630).
+
+Design.  One ``explicit_terms`` evaluation is one batched inverse transform of
+every field it needs (vorticity, divergence, temperature, tracers, u cos,
+v cos, grad ln ps, and the humidity extras), the nodal nonlinear terms, one
+batched forward transform of every nodal product, and a modal epilogue.  The
+reference instead transforms pytree leaf by leaf (pytree_utils.py:86-98).
+``implicit_terms`` and ``implicit_inverse`` run FP64 GPU kernels;
+``implicit_inverse`` factorises once per (levels, eta) through the vertical
+modes of A = G H + R T_ref dsigma^T (exact: the reference's per-l block
+inverse reduces to (I - eta^2 lambda_l A)^{-1} on the divergence).
+
+States may carry leading batch (ensemble member) dimensions: leaves are
+(..., K, 2M-1, L) tensors and every operation broadcasts over the leading
+dimensions.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import functools
+from typing import Any, Callable, Mapping
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import coordinate_systems
+from . import sigma_coordinates
+from . import spherical_harmonic
+from . import units as units_lib
+from ._lib import StateShapeError
+
+Array = Any
+
+
+# ------------------------------------------------------------------ State
+
+
+@dataclasses.dataclass
+class State:
+  """Spectral state of the primitive equations (primitive_equations.py:61-84)."""
+  vorticity: Array
+  divergence: Array
+  temperature_variation: Array
+  log_surface_pressure: Array
+  tracers: Mapping[str, Array] = dataclasses.field(default_factory=dict)
+  sim_time: float | None = None
+
+  _ARRAYS = ('vorticity', 'divergence', 'temperature_variation', 'log_surface_pressure')
+
+  def tree_map_arrays(self, fn):
+    kw = {f: fn(getattr(self, f)) for f in self._ARRAYS}
+    kw['tracers'] = {k: fn(v) for k, v in self.tracers.items()}
+    kw['sim_time'] = self.sim_time
+    return State(**kw)
+
+  def _combine(self, other, op):
+    if isinstance(other, State):
+      kw = {f: op(getattr(self, f), getattr(other, f)) for f in self._ARRAYS}
+      kw['tracers'] = {k: op(v, other.tracers[k]) for k, v in self.tracers.items()}
+      if self.sim_time is None or other.sim_time is None:
+        st = None
+      else:
+        st = op(self.sim_time, other.sim_time)
+      kw['sim_time'] = st
+      return State(**kw)
+    kw = {f: op(getattr(self, f), other) for f in self._ARRAYS}
+    kw['tracers'] = {k: op(v, other) for k, v in self.tracers.items()}
+    kw['sim_time'] = None if self.sim_time is None else op(self.sim_time, other)
+    return State(**kw)
+
+  def __add__(self, other):
+    return self._combine(other, lambda a, b: a + b)
+
+  def __sub__(self, other):
+    return self._combine(other, lambda a, b: a - b)
+
+  def __mul__(self, c):
+    return self._combine(c, lambda a, b: a * b)
+
+  __rmul__ = __mul__
+
+  def __neg__(self):
+    return self * -1.0
+
+  def asdict(self):
+    out = {f: getattr(self, f) for f in self._ARRAYS}
+    out['tracers'] = dict(self.tracers)
+    if self.sim_time is not None:
+      out['sim_time'] = self.sim_time
+    return out
+
+  @property
+  def fields(self):
+    return dataclasses.fields(self)
+
+
+def validate_state_shape(state: State, coords: coordinate_systems.CoordinateSystem):
+  """primitive_equations.py:91-120."""
+  for name in ('vorticity', 'divergence', 'temperature_variation'):
+    shape = tuple(getattr(state, name).shape[-3:])
+    if shape != coords.modal_shape:
+      raise StateShapeError(f'Expected {name} shape {coords.modal_shape}; got shape {shape}.')
+  shape = tuple(state.log_surface_pressure.shape[-3:])
+  if shape != coords.surface_modal_shape:
+    raise StateShapeError(f'Expected log_surface_pressure shape {coords.surface_modal_shape}; '
+                          f'got shape {shape}.')
+  for tracer_name, array in state.tracers.items():
+    if tuple(array.shape[-3:]) != coords.modal_shape:
+      raise StateShapeError(f'Expected tracer {tracer_name} shape {coords.modal_shape}; '
+                            f'got shape {tuple(array.shape)}.')
+
+
+# ------------------------------------------------------------------ vertical weights
+
+
+def get_sigma_ratios(coordinates: sigma_coordinates.SigmaCoordinates) -> np.ndarray:
+  """alpha[j] = log(s[j+1] / s[j]) / 2, alpha[-1] = -log(s[-1]) (:283-307)."""
+  alpha = np.diff(np.log(coordinates.centers), append=0) / 2
+  alpha[-1] = -np.log(coordinates.centers[-1])
+  return alpha
+
+
+def get_geopotential_weights_sigma(coordinates, ideal_gas_constant) -> np.ndarray:
+  """Upper-triangular G / R with alpha[j] on the diagonal and
+  alpha[k] + alpha[k-1] above it (:310-336)."""
+  alpha = get_sigma_ratios(coordinates)
+  k = coordinates.layers
+  # alpha[k] + alpha[k-1] in alpha's own dtype (float32 for float32 levels),
+  # as the reference's scalar loop does
+  pair = np.concatenate([[0.0], (alpha[1:] + alpha[:-1]).astype(np.float64)])
+  upper = np.triu(np.ones((k, k)), 1) * pair[None, :]
+  return ideal_gas_constant * (upper + np.diag(alpha.astype(np.float64)))
+
+
+def get_temperature_implicit_weights_sigma(coordinates, reference_temperature, kappa) -> np.ndarray:
+  """H (:418-487): kappa T_ref alpha terms minus the reference-lapse terms K."""
+  ref = np.asarray(reference_temperature)
+  if ref.ndim != 1 or ref.shape[-1] != coordinates.layers:
+    raise ValueError('`reference_temp` must be a vector of length `coordinates.layers`; '
+                     f'got shape {ref.shape} and {coordinates.layers} layers.')
+  k = coordinates.layers
+  lower = np.tril(np.ones((k, k)))
+  alpha = get_sigma_ratios(coordinates)[:, None]
+  pa = lower * alpha
+  pa_prev = np.zeros_like(pa)
+  pa_prev[1:] = pa[:-1]
+  ds = coordinates.layer_thickness
+  h0 = kappa * ref[:, None] * (pa + pa_prev) / ds[:, None]
+  k0 = np.concatenate((np.diff(ref) / (ds[:-1] + ds[1:]), [0]))[:, None]
+  kk = k0 * (lower - np.cumsum(ds)[:, None])
+  kk_prev = np.zeros_like(kk)
+  kk_prev[1:] = kk[:-1]
+  return (h0 - kk - kk_prev) * ds
+
+
+def _level_matrix(a: np.ndarray, x: torch.Tensor) -> torch.Tensor:
+  """'gh,...hml->...gml'."""
+  at = torch.as_tensor(np.asarray(a, np.float64), device=x.device).to(x.dtype)
+  return torch.einsum('gh,...hml->...gml', at, x)
+
+
+def get_geopotential_diff_sigma(temperature, coordinates, ideal_gas_constant, method='dense',
+                                sharding=None):
+  del method, sharding
+  return _level_matrix(get_geopotential_weights_sigma(coordinates, ideal_gas_constant),
+                       temperature)
+
+
+def get_temperature_implicit_sigma(divergence, coordinates, reference_temperature, kappa,
+                                   method='dense', sharding=None):
+  del method, sharding
+  return _level_matrix(
+      -get_temperature_implicit_weights_sigma(coordinates, reference_temperature, kappa),
+      divergence)
+
+
+def _get_implicit_term_matrix_sigma(eta, coords, reference_temperature, kappa,
+                                    ideal_gas_constant) -> np.ndarray:
+  """(L, 2K+1, 2K+1) FP64 matrix of (I - eta G) (:544-598); used for checks."""
+  k = coords.vertical.layers
+  lam = coords.horizontal.laplacian_eigenvalues
+  l = coords.horizontal.modal_shape[1]
+  g = get_geopotential_weights_sigma(coords.vertical, ideal_gas_constant)
+  h = get_temperature_implicit_weights_sigma(coords.vertical, reference_temperature, kappa)
+  t = np.asarray(reference_temperature, np.float64)
+  ds = coords.vertical.layer_thickness.astype(np.float64)
+  out = np.zeros((l, 2 * k + 1, 2 * k + 1))
+  out[:, :k, :k] = np.eye(k)
+  out[:, :k, k:2 * k] = eta * lam[:, None, None] * g[None]
+  out[:, :k, 2 * k] = eta * ideal_gas_constant * lam[:, None] * t[None, :]
+  out[:, k:2 * k, :k] = eta * h[None]
+  out[:, k:2 * k, k:2 * k] = np.eye(k)
+  out[:, 2 * k, :k] = eta * ds[None, :]
+  out[:, 2 * k, 2 * k] = 1.0
+  return out
+
+
+# ------------------------------------------------------------------ helpers
+
+
+def div_sec_lat(m_component, n_component, grid: spherical_harmonic.Grid):
+  """H(M, N) in the modal basis (:601-623)."""
+  sec2 = grid.const('sec2_lat', m_component.device, m_component.dtype)
+  mm, nn = grid.to_modal_batched([m_component * sec2, n_component * sec2])
+  return grid.div_cos_lat((mm, nn), clip=False)
+
+
+def truncated_modal_orography(orography, coords, wavenumbers_to_clip: int = 1):
+  """Modal orography with the top wavenumbers removed (:626-636)."""
+  grid = coords.horizontal
+  if tuple(orography.shape) != grid.nodal_shape:
+    raise ValueError(f'Expected nodal orography with shape={grid.nodal_shape}')
+  return grid.clip_wavenumbers(grid.to_modal(orography), n=wavenumbers_to_clip)
+
+
+def _vadv(w, x, dsig_c2c_inv, dtype_like):
+  """centered_vertical_advection along axis -3 with zero boundary fluxes."""
+  xd = torch.diff(x, dim=-3) * dsig_c2c_inv
+  wx = w * xd
+  wxp = torch.nn.functional.pad(wx, (0, 0, 0, 0, 1, 1))
+  return -0.5 * (wxp[..., 1:, :, :] + wxp[..., :-1, :, :])
+
+
+# ------------------------------------------------------------------ implicit factor
+
+
+class _ImplicitOperator:
+  """Holds the FP64 level matrices and per-eta GPU factor objects."""
+
+  def __init__(self, coords, reference_temperature, kappa, r_gas):
+    v = coords.vertical
+    self.K = v.layers
+    self.L = coords.horizontal.modal_shape[1]
+    self.R = coords.horizontal.modal_shape[0]
+    self.r_gas = float(r_gas)
+    self.G = np.ascontiguousarray(get_geopotential_weights_sigma(v, r_gas), np.float64)
+    self.H = np.ascontiguousarray(
+        get_temperature_implicit_weights_sigma(v, reference_temperature, kappa), np.float64)
+    self.tref = np.ascontiguousarray(np.asarray(reference_temperature, np.float64))
+    self.dsig = np.ascontiguousarray(v.layer_thickness.astype(np.float64))
+    self.lam = np.ascontiguousarray(coords.horizontal.laplacian_eigenvalues.astype(np.float64))
+    a = self.G @ self.H + self.r_gas * np.outer(self.tref, self.dsig)
+    d, vecs = np.linalg.eig(a)
+    if np.abs(d.imag).max() > 1e-12 * max(1.0, np.abs(d.real).max()):
+      raise ValueError('vertical structure matrix has complex eigenvalues; the reference '
+                       'temperature profile is outside the supported (stable) regime')
+    vecs = vecs.real
+    if np.linalg.cond(vecs) > 1e8:
+      raise ValueError('vertical modes are ill-conditioned (cond > 1e8)')
+    self.V = np.ascontiguousarray(vecs)
+    self.Vinv = np.ascontiguousarray(np.linalg.inv(vecs))
+    self.d = np.ascontiguousarray(d.real)
+    self._factors = {}
+
+  def factor(self, eta: float, device) -> ctypes.c_void_p:
+    key = (float(eta), torch.device(device).index)
+    f = self._factors.get(key)
+    if f is None:
+      denom = 1.0 - eta * eta * self.lam[:, None] * self.d[None, :]
+      if np.abs(denom).min() < 1e-12:
+        raise ValueError(f'implicit matrix is singular for step_size={eta}')
+      h = ctypes.c_void_p()
+      dp = _lib.dptr
+      with torch.cuda.device(device):
+        rc = _lib.lib().dino_implicit_create(
+            self.K, self.L, self.R, dp(self.G), dp(self.H), dp(self.tref), dp(self.dsig),
+            dp(self.lam), dp(self.V), dp(self.Vinv), dp(self.d), self.r_gas, float(eta),
+            ctypes.byref(h))
+      _lib.check(rc, 'dino_implicit_create')
+      f = h
+      self._factors[key] = f
+    return f
+
+  def __del__(self):
+    try:
+      if _lib._lib is None:
+        return
+      for h in getattr(self, '_factors', {}).values():
+        _lib._lib.dino_implicit_destroy(h)
+    except Exception:  # pylint: disable=broad-except  (interpreter shutdown)
+      pass
+
+  def apply(self, fn_name, eta, div, temp, lnps):
+    div = div.contiguous()
+    temp = temp.contiguous()
+    lnps = lnps.contiguous()
+    members = int(np.prod(div.shape[:-3])) if div.ndim > 3 else 1
+    f = self.factor(eta, div.device)
+    outs = [torch.empty_like(div), torch.empty_like(temp), torch.empty_like(lnps)]
+    code = _lib.DINO_F64 if div.dtype == torch.float64 else _lib.DINO_F32
+    if div.dtype not in (torch.float32, torch.float64):
+      raise TypeError(f'unsupported dtype {div.dtype}')
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    stream = ctypes.c_void_p(torch.cuda.current_stream(div.device).cuda_stream)
+    with torch.cuda.device(div.device):
+      rc = getattr(_lib.lib(), fn_name)(f, p(div), p(temp), p(lnps), p(outs[0]), p(outs[1]),
+                                        p(outs[2]), members, code, stream)
+    _lib.check(rc, fn_name)
+    return outs
+
+
+class _ImplicitSolve(torch.autograd.Function):
+  """(I - eta G)^{-1} on (div, temp, lnps); the adjoint applies the transposed
+  inverse, computed with the same kernel on the transposed factor."""
+
+  @staticmethod
+  def forward(ctx, div, temp, lnps, op, op_t, eta):
+    ctx.op_t, ctx.eta = op_t, eta
+    return tuple(op.apply('dino_implicit_solve', eta, div, temp, lnps))
+
+  @staticmethod
+  def backward(ctx, gd, gt, gl):
+    return tuple(ctx.op_t.apply_transpose(ctx.eta, gd, gt, gl)) + (None, None, None)
+
+
+class _ImplicitTerms(torch.autograd.Function):
+
+  @staticmethod
+  def forward(ctx, div, temp, lnps, op, eta):
+    ctx.op = op
+    return tuple(op.apply('dino_implicit_terms', eta, div, temp, lnps))
+
+  @staticmethod
+  def backward(ctx, gd, gt, gl):
+    return tuple(ctx.op.terms_transpose(gd, gt, gl)) + (None, None)
+
+
+def _implicit_transpose_ops(op: _ImplicitOperator):
+  """Adjoints of the implicit maps in torch FP64 level algebra.
+
+  terms: y_d = -lam (G T + R tref lnps), y_t = -H d, y_l = -ds . d
+  adjoint: d_bar = -H^T y_t - ds y_l ; T_bar = -G^T (lam y_d) ;
+           lnps_bar = -R tref . (lam y_d)
+  """
+
+  def terms_transpose(gd, gt, gl):
+    dev, dt = gd.device, gd.dtype
+    lam = torch.as_tensor(op.lam, device=dev, dtype=torch.float64)
+    g = torch.as_tensor(op.G, device=dev)
+    h = torch.as_tensor(op.H, device=dev)
+    tref = torch.as_tensor(op.tref, device=dev)
+    ds = torch.as_tensor(op.dsig, device=dev)
+    ld = gd.double() * lam
+    d_bar = -torch.einsum('hg,...hml->...gml', h, gt.double()) - ds[:, None, None] * gl.double()
+    t_bar = -torch.einsum('hg,...hml->...gml', g, ld)
+    l_bar = -op.r_gas * torch.einsum('h,...hml->...ml', tref, ld).unsqueeze(-3)
+    return d_bar.to(dt), t_bar.to(dt), l_bar.to(dt)
+
+  def apply_transpose(eta, gd, gt, gl):
+    # (M_l^{-1})^T = (M_l^T)^{-1}; solve with the dense FP64 per-l matrices.
+    dev, dt = gd.device, gd.dtype
+    mats = op.dense_inverse_t(eta, dev)  # (L, 2K+1, 2K+1) = inv(M_l)^T
+    k = op.K
+    x = torch.cat([gd.double(), gt.double(), gl.double()], dim=-3)  # (..., 2K+1, R, L)
+    y = torch.einsum('lgh,...hml->...gml', mats, x)
+    return y[..., :k, :, :].to(dt), y[..., k:2 * k, :, :].to(dt), y[..., 2 * k:, :, :].to(dt)
+
+  return terms_transpose, apply_transpose
+
+
+# ------------------------------------------------------------------ equations
+
+
+@dataclasses.dataclass
+class PrimitiveEquationsSigma:
+  """Primitive equations on sigma coordinates (primitive_equations.py:835-1458).
+
+  implicit_inverse_method, vertical_matmul_method: accepted for API
+  compatibility; every method gives the same operator here.
+  """
+  reference_temperature: np.ndarray
+  orography: Array
+  coords: coordinate_systems.CoordinateSystem
+  physics_specs: units_lib.SimUnits
+  vertical_matmul_method: str | None = dataclasses.field(default=None, kw_only=True)
+  implicit_inverse_method: str = dataclasses.field(default='split', kw_only=True)
+  vertical_advection: Callable | None = dataclasses.field(default=None, kw_only=True)
+  include_vertical_advection: bool = dataclasses.field(default=True, kw_only=True)
+  humidity_key: str | None = dataclasses.field(default=None, kw_only=True)
+  cloud_keys: tuple | None = dataclasses.field(default=None, kw_only=True)
+
+  def __post_init__(self):
+    if not np.allclose(self.coords.horizontal.radius, self.physics_specs.radius, rtol=1e-5):
+      raise ValueError('inconsistent radius between coordinates and constants: '
+                       f'{self.coords.horizontal.radius=} != {self.physics_specs.radius=}')
+    if self.cloud_keys is not None and self.humidity_key is None:
+      raise ValueError('cloud_keys requires humidity_key to be set.')
+    if self.implicit_inverse_method not in ('split', 'stacked', 'blockwise'):
+      raise ValueError(f'invalid implicit_inverse_method {self.implicit_inverse_method}')
+    self.reference_temperature = np.asarray(self.reference_temperature)
+    self._consts = {}
+
+  # ---- static data on the device
+  @functools.cached_property
+  def _implicit(self) -> _ImplicitOperator:
+    op = _ImplicitOperator(self.coords, self.reference_temperature,
+                           self.physics_specs.kappa, self.physics_specs.R)
+    terms_t, apply_t = _implicit_transpose_ops(op)
+    op.terms_transpose = terms_t
+    op.apply_transpose = apply_t
+    cache = {}
+
+    def dense_inverse_t(eta, dev):
+      key = (float(eta), dev)
+      if key not in cache:
+        m = _get_implicit_term_matrix_sigma(eta, self.coords, self.reference_temperature,
+                                            self.physics_specs.kappa, self.physics_specs.R)
+        cache[key] = torch.as_tensor(np.linalg.inv(m).transpose(0, 2, 1).copy(), device=dev)
+      return cache[key]
+    op.dense_inverse_t = dense_inverse_t
+    return op
+
+  def _c(self, name, like: torch.Tensor):
+    key = (name, like.device, like.dtype)
+    t = self._consts.get(key)
+    if t is None:
+      t = torch.as_tensor(np.asarray(self._host_const(name), np.float64),
+                          device=like.device).to(like.dtype)
+      self._consts[key] = t
+    return t
+
+  def _host_const(self, name):
+    grid = self.coords.horizontal
+    v = self.coords.vertical
+    if name == 'T_ref':
+      return self.reference_temperature[:, None, None]
+    if name == 'sec2':
+      return grid.sec2_lat
+    if name == 'coriolis':
+      return 2 * self.physics_specs.angular_velocity * grid.nodal_axes[1]
+    if name == 'dsig':
+      return v.layer_thickness[:, None, None]
+    if name == 'c2c_inv':
+      return (1 / v.center_to_center)[:, None, None]
+    if name == 'sum_sigma':
+      return np.cumsum(v.layer_thickness)[:-1, None, None]
+    if name == 'alpha':
+      return get_sigma_ratios(v)[:, None, None]
+    if name == 'geo_weights':
+      return get_geopotential_weights_sigma(v, self.physics_specs.R)
+    raise KeyError(name)
+
+  @property
+  def T_ref(self):
+    return self.reference_temperature[..., np.newaxis, np.newaxis]
+
+  @property
+  def coriolis_parameter(self):
+    _, sin_lat = self.coords.horizontal.nodal_mesh
+    return 2 * self.physics_specs.angular_velocity * sin_lat
+
+  # ---- explicit terms
+  def explicit_terms(self, state: State) -> State:
+    """Explicit tendencies (primitive_equations.py:1126-1194)."""
+    validate_state_shape(state, self.coords)
+    grid = self.coords.horizontal
+    ps = self.physics_specs
+    a = grid.radius
+    zeta, delta = state.vorticity, state.divergence
+    tv, lnps = state.temperature_variation, state.log_surface_pressure
+    x = zeta
+    moist = self.humidity_key is not None
+    tracer_names = list(state.tracers.keys())
+    if moist and self.humidity_key not in state.tracers:
+      raise ValueError(f'`{self.humidity_key}` is not found in tracers: {tracer_names}.')
+    if self.cloud_keys is not None:
+      for key in self.cloud_keys:
+        if key not in state.tracers:
+          raise ValueError(f'`{key}` is not found in tracers: {tracer_names}.')
+
+    # modal preparation (spherical_harmonic.py:1164-1178 with clip=False)
+    psi = grid.inverse_laplacian(zeta)
+    chi = grid.inverse_laplacian(delta)
+    ucos_m = (grid.d_dlon(chi) - grid.cos_lat_d_dlat(psi)) / a
+    vcos_m = (grid.cos_lat_d_dlat(chi) + grid.d_dlon(psi)) / a
+    gx_m = grid.d_dlon(lnps) / a
+    gy_m = grid.cos_lat_d_dlat(lnps) / a
+    inv = [zeta, delta, tv] + [state.tracers[k] for k in tracer_names] + [ucos_m, vcos_m, gx_m, gy_m]
+    if moist:
+      q_m = state.tracers[self.humidity_key]
+      inv += [grid.laplacian(lnps), grid.d_dlon(q_m) / a, grid.cos_lat_d_dlat(q_m) / a]
+    nod = grid.to_nodal_batched(inv)
+    zn, dn, tn = nod[0], nod[1], nod[2]
+    trn = dict(zip(tracer_names, nod[3:3 + len(tracer_names)]))
+    i0 = 3 + len(tracer_names)
+    un, vn, gxn, gyn = nod[i0:i0 + 4]
+    if moist:
+      lap_lnps_n, dqx_n, dqy_n = nod[i0 + 4:i0 + 7]
+
+    sec2 = self._c('sec2', x)
+    dsig = self._c('dsig', x)
+    c2c = self._c('c2c_inv', x)
+    tref = self._c('T_ref', x)
+    alpha = self._c('alpha', x)
+
+    # diagnostic state (primitive_equations.py:168-225)
+    udg = (un * gxn + vn * gyn) * sec2
+    f_expl = torch.cumsum(udg * dsig, dim=-3)
+    f_full = torch.cumsum((dn + udg) * dsig, dim=-3)
+    ss = self._c('sum_sigma', x)
+    sd_expl = ss * f_expl[..., -1:, :, :] - f_expl[..., :-1, :, :]
+    sd_full = ss * f_full[..., -1:, :, :] - f_full[..., :-1, :, :]
+
+    def vadv(w, field):
+      return _vadv(w, field, c2c, x)
+
+    # curl and divergence tendencies (:1001-1059)
+    total_vort = zn + self._c('coriolis', x)
+    nvu = -vn * total_vort * sec2
+    nvv = un * total_vort * sec2
+    if self.include_vertical_advection:
+      sdu = -vadv(sd_full, un)
+      sdv = -vadv(sd_full, vn)
+    else:
+      sdu = sdv = 0.0
+    if moist:
+      q = trn[self.humidity_key]
+      adj = 1 + (ps.R_vapor / ps.R - 1) * q
+      if self.cloud_keys is not None:
+        for key in self.cloud_keys:
+          adj = adj - trn[key]
+    else:
+      adj = 1.0
+    rt = ps.R * tn * adj
+    comb_u = nvu + (sdu + rt * gxn) * sec2
+    comb_v = nvv + (sdv + rt * gyn) * sec2
+    # kinetic energy (:778-785)
+    ke = (un * un + vn * vn) * sec2 / 2
+
+    # temperature tendencies (:1061-1115)
+    if self.include_vertical_advection:
+      dT_v = vadv(sd_full, tn)
+    else:
+      dT_v = 0.0
+    if np.unique(self.reference_temperature.ravel()).size > 1:
+      dT_v = dT_v + vadv(sd_expl, tref)
+    g_expl = udg
+    g_full = udg + dn
+
+    def t_omega(tfield, g_term):
+      f = torch.cumsum(g_term * dsig, dim=-3)
+      af = alpha * f
+      af_prev = torch.nn.functional.pad(af, (0, 0, 0, 0, 1, 0))[..., :-1, :, :]
+      return tfield * (udg - (af + af_prev) / dsig)
+
+    mean_t = t_omega(tref, g_expl)
+    if not moist:
+      dT_a = ps.kappa * (mean_t + t_omega(tn, g_full))
+    else:
+      gcr = ps.R_vapor / ps.R
+      hcr = ps.Cp_vapor / ps.Cp
+      vtc = tn * ((1 + (gcr - 1) * q) / (1 + (hcr - 1) * q))
+      hrc = tref * (((gcr - hcr) * q) / (1 + (hcr - 1) * q))
+      dT_a = ps.kappa * (mean_t + t_omega(vtc + hrc, g_full))
+    # log surface pressure (:1117-1124)
+    lsp_n = -(udg * dsig).sum(dim=-3, keepdim=True)
+    # horizontal advection of T and tracers (:795-805)
+    temp_n = tn * dn + dT_v + dT_a
+    fwd = [comb_u, comb_v, ke, temp_n, un * tn * sec2, vn * tn * sec2, lsp_n]
+    for k in tracer_names:
+      s = trn[k]
+      tr_v = vadv(sd_full, s) if self.include_vertical_advection else 0.0
+      fwd += [tr_v + s * dn, un * s * sec2, vn * s * sec2]
+    if moist:
+      coef = tref * (ps.R_vapor - ps.R)
+      gx1, gy1 = gxn, gyn
+      curl_term = coef * sec2 * (gx1 * dqy_n - gy1 * dqx_n)
+      corr = q * lap_lnps_n * tref * (ps.R_vapor - ps.R)
+      dot_term = coef * sec2 * (dqx_n * gx1 + dqy_n * gy1)
+      tdiff = q * (tn + tref) * (ps.R_vapor / ps.R - 1)
+      geo = torch.einsum('gh,...hij->...gij', self._c('geo_weights', x), tdiff)
+      fwd += [curl_term, geo, dot_term + corr]
+    mod = grid.to_modal_batched(fwd)
+    cu, cv, ke_m, temp_m, ut_m, vt_m, lsp_m = mod[:7]
+
+    def div_cos(mm, nn):
+      return (grid.d_dlon(mm) + grid.sec_lat_d_dlat_cos2(nn)) / a
+
+    def curl_cos(mm, nn):
+      return (grid.d_dlon(nn) - grid.sec_lat_d_dlat_cos2(mm)) / a
+
+    vort_t = -curl_cos(cu, cv)
+    div_t = -div_cos(cu, cv) - grid.laplacian(ke_m)
+    oro = torch.as_tensor(np.asarray(self.orography), device=x.device).to(x.dtype) \
+        if not isinstance(self.orography, torch.Tensor) else self.orography.to(x.device, x.dtype)
+    div_t = div_t - ps.g * grid.laplacian(oro)
+    temp_t = temp_m - div_cos(ut_m, vt_m)
+    tracers_t = {}
+    for n, k in enumerate(tracer_names):
+      tm, um, vm = mod[7 + 3 * n: 10 + 3 * n]
+      tracers_t[k] = tm - div_cos(um, vm)
+    if moist:
+      j0 = 7 + 3 * len(tracer_names)
+      curl_m, geo_m, dc_m = mod[j0:j0 + 3]
+      vort_t = vort_t + curl_m
+      div_t = div_t + (-grid.laplacian(geo_m) - dc_m)
+    tend = State(
+        vorticity=vort_t,
+        divergence=div_t,
+        temperature_variation=temp_t,
+        log_surface_pressure=lsp_m,
+        tracers=tracers_t,
+        sim_time=None if state.sim_time is None else 1.0,
+    )
+    return grid.clip_wavenumbers(tend)
+
+  # ---- implicit terms
+  def implicit_terms(self, state: State) -> State:
+    """Implicit tendencies (primitive_equations.py:1196-1249), FP64 kernel."""
+    validate_state_shape(state, self.coords)
+    op = self._implicit
+    d, t, l = _ImplicitTerms.apply(state.divergence, state.temperature_variation,
+                                   state.log_surface_pressure, op, 0.0)
+    return State(
+        vorticity=torch.zeros_like(state.vorticity),
+        divergence=d,
+        temperature_variation=t,
+        log_surface_pressure=l,
+        tracers={k: torch.zeros_like(v) for k, v in state.tracers.items()},
+        sim_time=None if state.sim_time is None else 0.0,
+    )
+
+  def implicit_inverse(self, state: State, step_size: float) -> State:
+    """(1 - step_size * implicit_terms)^{-1} (primitive_equations.py:1251-1458)."""
+    if isinstance(step_size, torch.Tensor):
+      raise TypeError(
+          f'`step_size` must be concrete but a tensor was passed: {step_size}. '
+          'Pass a Python float: the inverse is factorised in FP64 for each static step.')
+    validate_state_shape(state, self.coords)
+    op = self._implicit
+    d, t, l = _ImplicitSolve.apply(state.divergence, state.temperature_variation,
+                                   state.log_surface_pressure, op, op, float(step_size))
+    return State(state.vorticity, d, t, l, dict(state.tracers), sim_time=state.sim_time)
+
+  # ---- reference helper methods kept for API compatibility
+  def kinetic_energy_tendency(self, aux_state):
+    grid = self.coords.horizontal
+    u, v = aux_state.cos_lat_u
+    ke = (u * u + v * v) * self._c('sec2', u) / 2
+    return -grid.laplacian(grid.to_modal(ke))
+
+  def orography_tendency(self):
+    grid = self.coords.horizontal
+    oro = self.orography
+    return -self.physics_specs.g * grid.laplacian(oro)
+
+
+class PrimitiveEquations(PrimitiveEquationsSigma):
+  """Dry sigma-coordinate equations (deprecated alias, :2534-2562)."""
+
+  def __init__(self, reference_temperature, orography, coords, physics_specs, *,
+               vertical_matmul_method=None, implicit_inverse_method='split',
+               vertical_advection=None, include_vertical_advection=True):
+    super().__init__(reference_temperature, orography, coords, physics_specs,
+                     vertical_matmul_method=vertical_matmul_method,
+                     implicit_inverse_method=implicit_inverse_method,
+                     vertical_advection=vertical_advection,
+                     include_vertical_advection=include_vertical_advection,
+                     humidity_key=None, cloud_keys=None)
+
+
+PrimitiveEquationsWithTime = PrimitiveEquations
+
+
+class MoistPrimitiveEquations(PrimitiveEquationsSigma):
+  """Moist equations with specific humidity (deprecated alias, :2568-2596)."""
+
+  def __init__(self, reference_temperature, orography, coords, physics_specs, *,
+               vertical_matmul_method=None, implicit_inverse_method='split',
+               vertical_advection=None, include_vertical_advection=True):
+    super().__init__(reference_temperature, orography, coords, physics_specs,
+                     vertical_matmul_method=vertical_matmul_method,
+                     implicit_inverse_method=implicit_inverse_method,
+                     vertical_advection=vertical_advection,
+                     include_vertical_advection=include_vertical_advection,
+                     humidity_key='specific_humidity', cloud_keys=None)
+
+
+class MoistPrimitiveEquationsWithCloudMoisture(PrimitiveEquationsSigma):
+  """Moist equations with cloud water species (deprecated alias, :2599-2630)."""
+
+  def __init__(self, reference_temperature, orography, coords, physics_specs, *,
+               vertical_matmul_method=None, implicit_inverse_method='split',
+               vertical_advection=None, include_vertical_advection=True):
+    super().__init__(reference_temperature, orography, coords, physics_specs,
+                     vertical_matmul_method=vertical_matmul_method,
+                     implicit_inverse_method=implicit_inverse_method,
+                     vertical_advection=vertical_advection,
+                     include_vertical_advection=include_vertical_advection,
+                     humidity_key='specific_humidity',
+                     cloud_keys=('specific_cloud_liquid_water_content',
+                                 'specific_cloud_ice_water_content'))

@@ -115,12 +115,12 @@ class Plan:
     self.handle = handle
 
   def __del__(self):
-    h = getattr(self, 'handle', None)
-    if h is not None and h.value and _lib._lib is not None:
-      try:
+    try:
+      h = getattr(self, 'handle', None)
+      if h is not None and h.value and _lib._lib is not None:
         _lib._lib.dino_plan_destroy(h)
-      except Exception:  # interpreter shutdown
-        pass
+    except Exception:  # pylint: disable=broad-except  (interpreter shutdown)
+      pass
 
   def nodes(self):
     x = np.empty(self.J)

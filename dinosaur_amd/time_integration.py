@@ -1,0 +1,302 @@
+"""IMEX time stepping (mirrors dinosaur/time_integration.py:56-161, 313-637).
+
+Steppers are plain Python closures over torch operations, so a rollout can be
+recorded into a HIP graph with ``torch.cuda.graphs`` and differentiated with
+``torch.autograd`` (``checkpointed_rollout`` recomputes each segment of steps
+in the backward pass, the analogue of ``nested_checkpoint_scan``).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Any, Callable, Sequence
+
+import torch
+import torch.utils.checkpoint
+
+from . import filtering
+
+
+class ExplicitODE:
+  """time_integration.py:56-71."""
+
+  def explicit_terms(self, state):
+    raise NotImplementedError
+
+  @classmethod
+  def from_functions(cls, explicit_terms):
+    ode = cls()
+    ode.explicit_terms = explicit_terms
+    return ode
+
+
+class ImplicitExplicitODE:
+  """dx/dt = explicit_terms(x) + implicit_terms(x) (time_integration.py:74-114)."""
+
+  def explicit_terms(self, state):
+    raise NotImplementedError
+
+  def implicit_terms(self, state):
+    raise NotImplementedError
+
+  def implicit_inverse(self, state, step_size: float):
+    raise NotImplementedError
+
+  @classmethod
+  def from_functions(cls, explicit_terms, implicit_terms, implicit_inverse):
+    ode = cls()
+    ode.explicit_terms = explicit_terms
+    ode.implicit_terms = implicit_terms
+    ode.implicit_inverse = implicit_inverse
+    return ode
+
+
+def _is_imex(eq):
+  return hasattr(eq, 'implicit_inverse') and hasattr(eq, 'implicit_terms')
+
+
+def _tree_sum(items):
+  items = [x for x in items if x is not None]
+  out = items[0]
+  for x in items[1:]:
+    out = _add_trees(out, x)
+  return out
+
+
+def _add_trees(a, b):
+  if hasattr(a, '_ARRAYS'):  # primitive_equations.State
+    from .primitive_equations import State  # pylint: disable=g-import-not-at-top
+    kw = {f: getattr(a, f) + getattr(b, f) for f in a._ARRAYS}
+    kw['tracers'] = {k: v + b.tracers[k] if k in b.tracers else v for k, v in a.tracers.items()}
+    for k, v in b.tracers.items():
+      if k not in kw['tracers']:
+        kw['tracers'][k] = v
+    st = a.sim_time if a.sim_time is not None else b.sim_time
+    kw['sim_time'] = st
+    return State(**kw)
+  return a + b
+
+
+def compose_equations(equations: Sequence[Any]) -> ImplicitExplicitODE:
+  """Sums the explicit terms of several equations, keeping the single
+  implicit-explicit one for the implicit parts (time_integration.py:141-161)."""
+  imex = [e for e in equations if _is_imex(e)]
+  if len(imex) != 1:
+    raise ValueError('compose_equations supports at most 1 ImplicitExplicitODE '
+                     f'got {len(imex)}')
+  (main,) = imex
+
+  def explicit_fn(x):
+    return _tree_sum([e.explicit_terms(x) for e in equations])
+
+  return ImplicitExplicitODE.from_functions(explicit_fn, main.implicit_terms,
+                                            main.implicit_inverse)
+
+
+@dataclasses.dataclass
+class ImExButcherTableau:
+  a_ex: Sequence[Sequence[float]]
+  a_im: Sequence[Sequence[float]]
+  b_ex: Sequence[float]
+  b_im: Sequence[float]
+
+  def __post_init__(self):
+    if len({len(self.a_ex) + 1, len(self.a_im) + 1, len(self.b_ex), len(self.b_im)}) > 1:
+      raise ValueError('inconsistent Butcher tableau')
+
+
+def _weighted_sum(terms):
+  """sum(a * x) in the reference's order (time_integration.py:357-358)."""
+  acc = None
+  for a, x in terms:
+    t = a * x
+    acc = t if acc is None else acc + t
+  return acc
+
+
+def _stage(y0, dt, ex, im):
+  out = y0
+  for terms in (ex, im):
+    if terms:
+      out = out + dt * _weighted_sum(terms)
+  return out
+
+
+def imex_runge_kutta(tableau: ImExButcherTableau, equation, time_step: float):
+  """IMEX Runge-Kutta step (time_integration.py:333-370)."""
+  dt = time_step
+  F, G, G_inv = equation.explicit_terms, equation.implicit_terms, equation.implicit_inverse
+  a_ex, a_im, b_ex, b_im = tableau.a_ex, tableau.a_im, tableau.b_ex, tableau.b_im
+  n = len(b_ex)
+
+  def step_fn(y0):
+    f = [None] * n
+    g = [None] * n
+    f[0] = F(y0)
+    g[0] = G(y0)
+    for i in range(1, n):
+      ex = [(a_ex[i - 1][j], f[j]) for j in range(i) if a_ex[i - 1][j]]
+      im = [(a_im[i - 1][j], g[j]) for j in range(i) if a_im[i - 1][j]]
+      y_star = _stage(y0, dt, ex, im)
+      y = G_inv(y_star, dt * a_im[i - 1][i])
+      if any(a_ex[j][i] for j in range(i, n - 1)) or b_ex[i]:
+        f[i] = F(y)
+      if any(a_im[j][i] for j in range(i, n - 1)) or b_im[i]:
+        g[i] = G(y)
+    ex = [(b_ex[j], f[j]) for j in range(n) if b_ex[j]]
+    im = [(b_im[j], g[j]) for j in range(n) if b_im[j]]
+    return _stage(y0, dt, ex, im)
+
+  return step_fn
+
+
+SIL3_TABLEAU = ImExButcherTableau(
+    a_ex=[[1 / 3], [1 / 6, 1 / 2], [1 / 2, -1 / 2, 1]],
+    a_im=[[1 / 6, 1 / 6], [1 / 3, 0, 1 / 3], [3 / 8, 0, 3 / 8, 1 / 4]],
+    b_ex=[1 / 2, -1 / 2, 1, 0],
+    b_im=[3 / 8, 0, 3 / 8, 1 / 4],
+)
+
+
+def imex_rk_sil3(equation, time_step: float):
+  """Whitaker & Kar (2013) SIL3 (time_integration.py:373-405)."""
+  return imex_runge_kutta(SIL3_TABLEAU, equation, time_step)
+
+
+def runge_kutta_step_filter(state_filter):
+  def _filter(u, u_next):
+    del u
+    return state_filter(u_next)
+  return _filter
+
+
+def exponential_step_filter(grid, dt: float, tau: float = 0.010938, order: int = 18,
+                            cutoff: float = 0):
+  """time_integration.py:427-455."""
+  return runge_kutta_step_filter(filtering.exponential_filter(grid, dt / tau, order, cutoff))
+
+
+def horizontal_diffusion_step_filter(grid, dt: float, tau: float, order: int = 1):
+  """time_integration.py:459-486."""
+  eigenvalues = grid.laplacian_eigenvalues
+  scale = dt / (tau * abs(eigenvalues[-1]) ** order)
+  return runge_kutta_step_filter(filtering.horizontal_diffusion_filter(grid, scale, order))
+
+
+def step_with_filters(step_fn, filters):
+  def _step_fn(u):
+    u_next = step_fn(u)
+    for filter_fn in filters:
+      u_next = filter_fn(u, u_next)
+    return u_next
+  return _step_fn
+
+
+def repeated(fn, steps: int, scan_fn=None):
+  if steps == 1:
+    return fn
+
+  def f_repeated(x):
+    for _ in range(steps):
+      x = fn(x)
+    return x
+  return f_repeated
+
+
+def trajectory_from_step(step_fn, outer_steps: int, inner_steps: int, *,
+                         start_with_input: bool = False, post_process_fn=lambda x: x,
+                         outer_scan_fn=None, inner_scan_fn=None):
+  """Returns x -> (final state, list of outer frames) (time_integration.py:523-564)."""
+  inner = repeated(step_fn, inner_steps)
+
+  def multistep(x):
+    frames = []
+    for _ in range(outer_steps):
+      y = inner(x)
+      frames.append(post_process_fn(x if start_with_input else y))
+      x = y
+    return x, frames
+  return multistep
+
+
+def _flatten_state(x):
+  """State -> (list of tensors, rebuild)."""
+  if hasattr(x, '_ARRAYS'):
+    names = list(x._ARRAYS)
+    keys = list(x.tracers.keys())
+    leaves = [getattr(x, n) for n in names] + [x.tracers[k] for k in keys]
+    st = x.sim_time
+    cls = type(x)
+
+    def rebuild(ls):
+      kw = dict(zip(names, ls[:len(names)]))
+      kw['tracers'] = dict(zip(keys, ls[len(names):]))
+      kw['sim_time'] = st
+      return cls(**kw)
+    return leaves, rebuild
+  if isinstance(x, torch.Tensor):
+    return [x], lambda ls: ls[0]
+  raise TypeError(f'cannot flatten {type(x)}')
+
+
+def checkpointed_rollout(step_fn, x, steps: int, checkpoint_every: int):
+  """Runs `steps` steps keeping only every `checkpoint_every`-th state for the
+  backward pass; each segment is recomputed when gradients are taken (the
+  role of nested_checkpoint_scan, time_integration.py:573-637)."""
+  if steps % checkpoint_every:
+    raise ValueError(f'inconsistent steps={steps} and checkpoint_every={checkpoint_every}')
+  leaves, rebuild = _flatten_state(x)
+
+  def segment(*ls):
+    y = rebuild(list(ls))
+    for _ in range(checkpoint_every):
+      y = step_fn(y)
+    out, _ = _flatten_state(y)
+    return tuple(out)
+
+  for _ in range(steps // checkpoint_every):
+    if torch.is_grad_enabled() and any(t.requires_grad for t in leaves):
+      leaves = list(torch.utils.checkpoint.checkpoint(segment, *leaves, use_reentrant=False))
+    else:
+      leaves = list(segment(*leaves))
+  return rebuild(leaves)
+
+
+def nested_checkpoint_scan(f, init, xs, length=None, *, nested_lengths, scan_fn=None,
+                           checkpoint_fn=None):
+  """scan with recursive checkpointing (time_integration.py:573-637).
+
+  ``f(carry, x) -> (carry, y)``; xs is None or a sequence of length
+  prod(nested_lengths).  Returns (carry, list of outputs)."""
+  total = math.prod(nested_lengths)
+  if length is not None and length != total:
+    raise ValueError(f'inconsistent {length=} and {nested_lengths=}')
+  xs_list = [None] * total if xs is None else list(xs)
+
+  def run(carry, lo, lengths):
+    if len(lengths) == 1:
+      outs = []
+      for i in range(lengths[0]):
+        carry, y = f(carry, xs_list[lo + i])
+        outs.append(y)
+      return carry, outs
+    sub = math.prod(lengths[1:])
+    outs = []
+    for i in range(lengths[0]):
+      start = lo + i * sub
+      leaves, rebuild = _flatten_state(carry)
+
+      def seg(*ls, start=start, rebuild=rebuild):
+        c, o = run(rebuild(list(ls)), start, lengths[1:])
+        cl, _ = _flatten_state(c)
+        seg.outs = o
+        return tuple(cl)
+      if torch.is_grad_enabled() and any(t.requires_grad for t in leaves):
+        new = torch.utils.checkpoint.checkpoint(seg, *leaves, use_reentrant=False)
+      else:
+        new = seg(*leaves)
+      carry = rebuild(list(new))
+      outs.extend(seg.outs)
+    return carry, outs
+
+  return run(init, 0, list(nested_lengths))

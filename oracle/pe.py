@@ -487,7 +487,9 @@ class HeldSuarezForcingSigma:
     self.maxT = 315.0 if maxT is None else maxT
     self.dTy = 60.0 if dTy is None else dTy
     self.dThz = 10.0 if dThz is None else dThz
-    self.sigma = coords.vertical.centers.astype(np.float64)
+    # float32 centres, as the reference (held_suarez.py:96): kv and kt are
+    # evaluated in the centres' dtype
+    self.sigma = coords.vertical.centers
     _, sin_lat = coords.horizontal.nodal_mesh
     self.lat = np.arcsin(sin_lat)
 

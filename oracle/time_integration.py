@@ -36,12 +36,16 @@ SIL3 = Tableau(
 )
 
 
-def _lin(terms):
-  """sum(c * x) over (c, x) pairs for State or ndarray."""
-  out = None
-  for c, x in terms:
-    t = x * c
-    out = t if out is None else out + t
+def _stage(y0, dt, ex, im):
+  """y0 + dt * sum(a * f) + dt * sum(a * g) (time_integration.py:355-369)."""
+  out = y0
+  for terms in (ex, im):
+    if terms:
+      acc = None
+      for a, x in terms:
+        t = a * x
+        acc = t if acc is None else acc + t
+      out = out + dt * acc
   return out
 
 
@@ -58,17 +62,17 @@ def imex_runge_kutta(tableau, equation, time_step):
     f[0] = F(y0)
     g[0] = G(y0)
     for i in range(1, n):
-      ex = [(dt * a_ex[i - 1][j], f[j]) for j in range(i) if a_ex[i - 1][j]]
-      im = [(dt * a_im[i - 1][j], g[j]) for j in range(i) if a_im[i - 1][j]]
-      y_star = _lin([(1.0, y0)] + ex + im)
+      ex = [(a_ex[i - 1][j], f[j]) for j in range(i) if a_ex[i - 1][j]]
+      im = [(a_im[i - 1][j], g[j]) for j in range(i) if a_im[i - 1][j]]
+      y_star = _stage(y0, dt, ex, im)
       y = G_inv(y_star, dt * a_im[i - 1][i])
       if any(a_ex[j][i] for j in range(i, n - 1)) or b_ex[i]:
         f[i] = F(y)
       if any(a_im[j][i] for j in range(i, n - 1)) or b_im[i]:
         g[i] = G(y)
-    ex = [(dt * b_ex[j], f[j]) for j in range(n) if b_ex[j]]
-    im = [(dt * b_im[j], g[j]) for j in range(n) if b_im[j]]
-    return _lin([(1.0, y0)] + ex + im)
+    ex = [(b_ex[j], f[j]) for j in range(n) if b_ex[j]]
+    im = [(b_im[j], g[j]) for j in range(n) if b_im[j]]
+    return _stage(y0, dt, ex, im)
 
   return step_fn
 

@@ -154,7 +154,7 @@ def test_held_suarez_closed_forms():
   hs = pe.HeldSuarezForcingSigma(coords, PS, reft)
   s = coords.vertical.centers.astype(np.float64)
   kv = hs.kv()[:, 0, 0]
-  np.testing.assert_allclose(kv, units.per_day(1) * np.maximum(0, (s - 0.7) / 0.3))
+  np.testing.assert_allclose(kv, units.per_day(1) * np.maximum(0, (s - 0.7) / 0.3), rtol=1e-6)
   out = hs.explicit_terms(st)
   assert np.all(out.log_surface_pressure == 0)
   # at rest: only Newtonian cooling toward T_eq acts

@@ -1,0 +1,345 @@
+#!/usr/bin/env python
+"""Benchmark: member-simulated-days per second of a TL255 L37 moist ensemble,
+forward plus checkpointed adjoint, on one MI355X.
+
+Workload (the full chosen-form configuration): TL255 (512 x 256 Gaussian grid,
+2M-1 = 511 x L = 257 modal), 37 equidistant sigma levels, moist primitive
+equations with specific humidity and two cloud tracers, 16 members, SIL3 with
+the exponential step filter at dt = 5 min, seeded synthetic initial states
+(rest atmosphere over seeded 2 km orography + seeded perturbations and
+humidity).  One benchmark step = one SIL3 step of the whole ensemble, forward
+and its adjoint (the rollout keeps a checkpoint per step and recomputes it in
+the backward pass).  The loss is quadratic in the final level-18 (sigma ~ 0.5)
+nodal temperature and vorticity.
+
+Legs, all in one run:
+* chosen   - the dinosaur_amd HIP pipeline (value of the JSON line);
+* plain    - the reference's formulation in plain FP32 on the same GPU:
+             dense (2M-1, J, L) Legendre and (I, 2M-1) Fourier matmuls
+             through torch and the nodal terms as separate tensor ops,
+             otherwise the same equations and rollout;
+* cpu      - the oracle's FP32 dense transforms on the host cores, a bounded
+             sample extrapolated to one member-step (forward only).
+The roofline entry times the dominant kernel live with HIP events on the
+stream the kernel runs on.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+MEMBERS = 16
+GRID = 'TL255'
+LAYERS = 37
+DT_SECONDS = 300.0
+
+
+def _parse():
+  ap = argparse.ArgumentParser()
+  ap.add_argument('--gpus', type=int, default=1)
+  ap.add_argument('--steps', type=int, default=4)
+  ap.add_argument('--warmup', type=int, default=1)
+  ap.add_argument('--members', type=int, default=MEMBERS)
+  ap.add_argument('--precision', default='auto', help='float32 | f16x3 | bf16x3 | auto')
+  ap.add_argument('--plain-steps', type=int, default=1)
+  ap.add_argument('--no-plain', action='store_true')
+  ap.add_argument('--no-cpu', action='store_true')
+  ap.add_argument('--forward-only', action='store_true')
+  return ap.parse_args()
+
+
+def _sync(torch):
+  torch.cuda.synchronize()
+
+
+def build(torch, precision, members, impl=None, fused_nodal=True):
+  from dinosaur_amd import time_integration as ti
+  from dinosaur_amd import units as units_lib
+  from dinosaur_amd import workloads
+  eq, state, coords, ps = workloads.build_primitive(
+      GRID, LAYERS, moist=True, members=members, transform_precision=precision,
+      impl=impl, fused_nodal=fused_nodal)
+  dt = ps.nondimensionalize(DT_SECONDS * units_lib.scales.units.s)
+  step = ti.step_with_filters(ti.imex_rk_sil3(eq, dt),
+                              [ti.exponential_step_filter(coords.horizontal, dt)])
+  rng = np.random.default_rng(7)
+  grid = coords.horizontal
+  t18 = grid.to_nodal(state.temperature_variation[..., 18, :, :])
+  z18 = grid.to_nodal(state.vorticity[..., 18, :, :])
+  targets = (t18 + torch.as_tensor(rng.normal(0, 0.5, t18.shape), device=t18.device,
+                                   dtype=t18.dtype),
+             z18 + torch.as_tensor(rng.normal(0, 1e-6, z18.shape), device=z18.device,
+                                   dtype=z18.dtype))
+  return eq, state, coords, step, targets
+
+
+def rollout_fwd_adj(torch, step, state, coords, targets, steps, forward_only=False):
+  from dinosaur_amd import primitive_equations as pe
+  from dinosaur_amd import time_integration as ti
+  grid = coords.horizontal
+  leaves = {f: getattr(state, f).detach().clone().requires_grad_(not forward_only)
+            for f in pe.State._ARRAYS}
+  tracers = {k: v.detach().clone().requires_grad_(not forward_only)
+             for k, v in state.tracers.items()}
+  s0 = pe.State(**leaves, tracers=tracers)
+  if forward_only:
+    with torch.no_grad():
+      y = s0
+      for _ in range(steps):
+        y = step(y)
+    return None
+  y = ti.checkpointed_rollout(step, s0, steps, checkpoint_every=1)
+  t18 = grid.to_nodal(y.temperature_variation[..., 18, :, :])
+  z18 = grid.to_nodal(y.vorticity[..., 18, :, :])
+  loss = ((t18 - targets[0]) ** 2).mean() + 1e12 * ((z18 - targets[1]) ** 2).mean()
+  grads = torch.autograd.grad(loss, list(leaves.values()) + list(tracers.values()))
+  return loss.detach(), grads
+
+
+def time_leg(torch, step, state, coords, targets, steps, warmup, forward_only=False):
+  for _ in range(warmup):
+    rollout_fwd_adj(torch, step, state, coords, targets, 1, forward_only)
+  _sync(torch)
+  t0 = time.perf_counter()
+  out = rollout_fwd_adj(torch, step, state, coords, targets, steps, forward_only)
+  _sync(torch)
+  wall = time.perf_counter() - t0
+  return wall, out
+
+
+class DenseFP32Harmonics:
+  """Factory for the reference formulation of the transforms (dense einsums,
+  spherical_harmonic.py:262-280), executed by torch matmuls in FP32."""
+
+  def __new__(cls, **kw):
+    import torch
+    from dinosaur_amd import spherical_harmonic as sh
+
+    class _Dense(sh.RealSphericalHarmonics):
+      _cache = {}
+
+      def _tables(self, device, dtype):
+        key = (self.longitude_wavenumbers, self.total_wavenumbers, self.longitude_nodes,
+               self.latitude_nodes, device, dtype)
+        t = _Dense._cache.get(key)
+        if t is None:
+          b = self.basis
+          f = torch.as_tensor(b.f, device=device, dtype=dtype)
+          p = torch.as_tensor(b.p, device=device, dtype=dtype)
+          w = torch.as_tensor(b.w, device=device, dtype=dtype)
+          t = (f, f.t().contiguous(), p.contiguous(), p.transpose(1, 2).contiguous(), w)
+          _Dense._cache[key] = t
+        return t
+
+      def inverse_transform(self, x):
+        f, ft, p, pt, w = self._tables(x.device, x.dtype)
+        lead = x.shape[:-2]
+        R, L = self.modal_shape
+        I, J = self.nodal_shape
+        xb = x.reshape(-1, R, L).transpose(0, 1).contiguous()          # (R, B, L)
+        px = torch.bmm(xb, pt)                                          # (R, B, J)
+        B = px.shape[1]
+        y = (f @ px.reshape(R, B * J)).reshape(I, B, J).transpose(0, 1)  # (B, I, J)
+        return y.reshape(lead + (I, J))
+
+      def transform(self, z):
+        f, ft, p, pt, w = self._tables(z.device, z.dtype)
+        lead = z.shape[:-2]
+        R, L = self.modal_shape
+        I, J = self.nodal_shape
+        zb = (z.reshape(-1, I, J) * w).transpose(0, 1).contiguous()    # (I, B, J)
+        B = zb.shape[1]
+        fz = (ft @ zb.reshape(I, B * J)).reshape(R, B, J)              # (R, B, J)
+        y = torch.bmm(fz, p)                                            # (R, B, L)
+        return y.transpose(0, 1).reshape(lead + (R, L))
+
+    return _Dense(**kw)
+
+
+def measure_kernel(torch, coords, batch, stage, precision_code, reps=10):
+  """Average duration of one stage kernel launch (HIP events on the launch
+  stream) and its algorithmic work."""
+  from dinosaur_amd import _lib
+  grid = coords.horizontal
+  sh = grid.spherical_harmonics
+  dev = torch.device('cuda', torch.cuda.current_device())
+  plan = sh.plan(dev)
+  M, L, I, J, R = plan.M, plan.L, plan.I, plan.J, plan.R
+  g = torch.Generator(device=dev).manual_seed(0)
+  modal = torch.randn((batch, R, L), device=dev, generator=g)
+  fourier = torch.randn((batch, R, J), device=dev, generator=g)
+  nodal = torch.randn((batch, I, J), device=dev, generator=g)
+  lib = _lib.lib()
+  stream = torch.cuda.current_stream(dev)
+  sptr = ctypes.c_void_p(stream.cuda_stream)
+  P = lambda t: ctypes.c_void_p(t.data_ptr())
+  if stage == 'legendre_inverse':
+    call = lambda: lib.dino_legendre_inverse(plan.handle, P(modal), P(fourier), batch, 0,
+                                             precision_code, sptr)
+  elif stage == 'legendre_forward':
+    call = lambda: lib.dino_legendre_forward(plan.handle, P(fourier), P(modal), batch, 0,
+                                             precision_code, sptr)
+  elif stage == 'dft_forward':
+    call = lambda: lib.dino_dft_forward(plan.handle, P(nodal), P(fourier), batch, 0, 1, sptr)
+  else:
+    call = lambda: lib.dino_dft_inverse(plan.handle, P(fourier), P(nodal), batch, 0, 0, sptr)
+  _lib.check(call(), stage)
+  torch.cuda.synchronize()
+  start = torch.cuda.Event(enable_timing=True)
+  end = torch.cuda.Event(enable_timing=True)
+  start.record(stream)
+  for _ in range(reps):
+    call()
+  end.record(stream)
+  end.synchronize()
+  secs = start.elapsed_time(end) / 1e3 / reps
+  n_tri = M * L - M * (M - 1) // 2
+  n_real = 2 * n_tri - L
+  if stage.startswith('legendre'):
+    work = float(J) * n_real * batch  # folded triangular flops
+    return secs, work, 'flop'
+  work = 4.0 * (I * J + R * J) * batch
+  return secs, work, 'byte'
+
+
+def cpu_baseline(sample_fields=37):
+  """Oracle FP32 dense transforms on the host cores, extrapolated to one
+  member-step of the workload (forward only)."""
+  import torch
+  sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+  from oracle import sht as osht
+  g = osht.Grid.TL255()
+  g._basis_as(np.float32)
+  rng = np.random.default_rng(0)
+  x = rng.standard_normal((sample_fields,) + g.modal_shape).astype(np.float32)
+  t0 = time.perf_counter()
+  y = g.to_nodal(x)
+  t1 = time.perf_counter()
+  g.to_modal(y)
+  t2 = time.perf_counter()
+  per_inv = (t1 - t0) / sample_fields
+  per_fwd = (t2 - t1) / sample_fields
+  n_inv, n_fwd = 374, 667  # field transforms per explicit evaluation (L37, 3 tracers)
+  per_member_step = 3 * (n_inv * per_inv + n_fwd * per_fwd)
+  cores = torch.get_num_threads()
+  try:
+    cores = len(os.sched_getaffinity(0))
+  except AttributeError:
+    pass
+  return {
+      'value': (DT_SECONDS / 86400.0) / per_member_step,
+      'unit': 'member-sim-days/s',
+      'cores': cores,
+      'kind': 'port',
+      'sample': (f'oracle FP32 dense-einsum TL255 transforms of {sample_fields} fields each '
+                 f'way ({t2 - t0:.1f} s), extrapolated to 3 x ({n_inv} inverse + {n_fwd} '
+                 'forward) field transforms per member-step; forward only, nodal terms '
+                 f'excluded; host {platform.processor() or platform.machine()}'),
+  }
+
+
+def main():
+  args = _parse()
+  if args.gpus != 1:
+    print(json.dumps({'error': 'bench.py measures one GPU; multi-GPU runs are not part of '
+                      'this build (the ensemble would shard by member with no exchange)'}))
+    sys.exit(2)
+  import torch
+  from dinosaur_amd import _lib
+  torch.backends.cuda.matmul.allow_tf32 = False
+  dev = torch.device('cuda', 0)
+  torch.cuda.set_device(dev)
+  precision = args.precision
+  if precision == 'auto':
+    precision = 'f16x3' if _lib.lib() and _split_available(torch) else 'float32'
+  members = args.members
+  print(f'[bench] chosen leg: {GRID} L{LAYERS} moist+3 tracers, {members} members, '
+        f'precision={precision}', file=sys.stderr, flush=True)
+  eq, state, coords, step, targets = build(torch, precision, members)
+  wall, _ = time_leg(torch, step, state, coords, targets, args.steps, args.warmup,
+                     args.forward_only)
+  days = DT_SECONDS / 86400.0
+  value = members * args.steps * days / wall
+  ms_per_step = wall / args.steps * 1e3
+  print(f'[bench] chosen: {wall:.3f} s for {args.steps} steps, {ms_per_step:.1f} ms/step, '
+        f'{value:.3f} member-sim-days/s', file=sys.stderr, flush=True)
+
+  # dominant kernel, timed live on its launch stream
+  pcode = _lib.PRECISIONS[precision]
+  fwd_batch = members * 667
+  kern = {}
+  for stage in ('legendre_forward', 'legendre_inverse', 'dft_forward', 'dft_inverse'):
+    secs, work, kind = measure_kernel(torch, coords, fwd_batch if stage.endswith('forward')
+                                      else members * 374, stage, pcode)
+    kern[stage] = (secs, work, kind)
+    print(f'[bench] {stage}: {secs * 1e3:.3f} ms/launch', file=sys.stderr, flush=True)
+  dom = max(kern, key=lambda k: kern[k][0])
+  secs, work, kind = kern[dom]
+  if kind == 'flop':
+    peak = 157.3 if precision == 'float32' else 2500.0
+    roof = {'bound': 'mfma', 'achieved': work / secs / 1e12, 'peak': peak, 'unit': 'TFLOP/s'}
+  else:
+    roof = {'bound': 'hbm', 'achieved': work / secs / 1e9, 'peak': 8000.0, 'unit': 'GB/s'}
+  roof['frac'] = roof['achieved'] / roof['peak']
+  roof['traffic'] = None
+  roof['kernel'] = dom
+
+  plain = None
+  del eq, state, step, targets
+  torch.cuda.empty_cache()
+  if not args.no_plain:
+    print('[bench] plain FP32 leg (dense reference formulation)', file=sys.stderr, flush=True)
+    _, pstate, pcoords, pstep, ptargets = build(torch, 'float32', members,
+                                                impl=DenseFP32Harmonics, fused_nodal=False)
+    pwall, _ = time_leg(torch, pstep, pstate, pcoords, ptargets, args.plain_steps, 1,
+                        args.forward_only)
+    pval = members * args.plain_steps * days / pwall
+    plain = {'value': pval, 'ms_per_step': pwall / args.plain_steps * 1e3,
+             'steps': args.plain_steps}
+    print(f'[bench] plain: {pwall:.3f} s, {pval:.3f} member-sim-days/s', file=sys.stderr,
+          flush=True)
+    del pstate, pstep, ptargets
+    torch.cuda.empty_cache()
+  cpu = None if args.no_cpu else cpu_baseline()
+  out = {
+      'metric': 'TL255 L37 sim-days/s fwd+adjoint, chosen vs plain FP32, <=1e-5 rel err vs FP64',
+      'value': value,
+      'unit': 'member-sim-days/s',
+      'n_gpus': 1,
+      'steps': args.steps,
+      'warmup': args.warmup,
+      'ms_per_step': ms_per_step,
+      'higher_is_better': True,
+      'scaling': 'weak',
+      'vs_baseline': None,
+      'dtype': 'f32' if precision == 'float32' else f'f32 ({precision} Legendre)',
+      'data': 'synthetic (seeded rest atmosphere + orography + perturbations + humidity)',
+      'config': {'workload': f'{GRID} L{LAYERS} moist (q + 2 clouds), {members} members, '
+                             'SIL3 dt=5 min, forward + checkpointed adjoint',
+                 'members': members, 'grid': GRID, 'levels': LAYERS,
+                 'ensemble_sim_days_per_s': args.steps * days / wall,
+                 'plain_fp32': plain,
+                 'chosen_over_plain': (value / plain['value']) if plain else None,
+                 'forward_only': args.forward_only},
+      'roofline': roof,
+      'cpu_baseline': cpu,
+  }
+  print(json.dumps(out))
+
+
+def _split_available(torch):
+  from dinosaur_amd import _lib
+  import ctypes as C
+  lib = _lib.lib()
+  return bool(getattr(lib, 'dino_version')) and b'split' in lib.dino_version()
+
+
+if __name__ == '__main__':
+  main()

@@ -51,6 +51,13 @@ SIGNATURES = {
     'dino_implicit_destroy': (_c_i, [_c_p]),
     'dino_implicit_solve': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
     'dino_implicit_terms': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
+    'dino_nodal_create': (_c_i, [_c_i, _c_i, _c_i, _c_i, _c_i, ctypes.POINTER(_c_i), _c_i, _c_i,
+                                 _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp,
+                                 _c_d, _c_d, _c_d, _c_d, _c_d, ctypes.POINTER(_c_p)]),
+    'dino_nodal_destroy': (_c_i, [_c_p]),
+    'dino_nodal_fields': (_c_i64, [_c_p, _c_i64, _c_i]),
+    'dino_nodal_explicit': (_c_i, [_c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
+    'dino_nodal_explicit_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
 }
 
 _lib = None

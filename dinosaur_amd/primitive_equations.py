@@ -367,6 +367,58 @@ def _implicit_transpose_ops(op: _ImplicitOperator):
   return terms_transpose, apply_transpose
 
 
+# ------------------------------------------------------------------ fused nodal terms
+
+
+class _NodalHandle:
+  def __init__(self, handle):
+    self.handle = handle
+
+  def __del__(self):
+    try:
+      if _lib._lib is not None and self.handle.value:
+        _lib._lib.dino_nodal_destroy(self.handle)
+    except Exception:  # pylint: disable=broad-except  (interpreter shutdown)
+      pass
+
+
+class _NodalFused(torch.autograd.Function):
+  """(fields_in, I, J) nodal inputs -> (fields_out, I, J) nodal products, with
+  the hand-written adjoint kernel as backward."""
+
+  @staticmethod
+  def forward(ctx, nodal, h, members):
+    nodal = nodal.contiguous()
+    lib = _lib.lib()
+    n_out = lib.dino_nodal_fields(h.handle, members, 1)
+    n_in = lib.dino_nodal_fields(h.handle, members, 0)
+    if nodal.shape[0] != n_in:
+      raise ValueError(f'fused nodal kernel expects {n_in} input fields, got {nodal.shape[0]}')
+    out = torch.empty((n_out,) + tuple(nodal.shape[1:]), dtype=nodal.dtype, device=nodal.device)
+    code = _lib.DINO_F64 if nodal.dtype == torch.float64 else _lib.DINO_F32
+    stream = ctypes.c_void_p(torch.cuda.current_stream(nodal.device).cuda_stream)
+    with torch.cuda.device(nodal.device):
+      rc = lib.dino_nodal_explicit(h.handle, ctypes.c_void_p(nodal.data_ptr()),
+                                   ctypes.c_void_p(out.data_ptr()), members, code, stream)
+    _lib.check(rc, 'dino_nodal_explicit')
+    ctx.save_for_backward(nodal)
+    ctx.h, ctx.members, ctx.code = h, members, code
+    return out
+
+  @staticmethod
+  def backward(ctx, g):
+    (nodal,) = ctx.saved_tensors
+    g = g.contiguous()
+    gin = torch.empty_like(nodal)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(nodal.device).cuda_stream)
+    with torch.cuda.device(nodal.device):
+      rc = _lib.lib().dino_nodal_explicit_adjoint(
+          ctx.h.handle, ctypes.c_void_p(nodal.data_ptr()), ctypes.c_void_p(g.data_ptr()),
+          ctypes.c_void_p(gin.data_ptr()), ctx.members, ctx.code, stream)
+    _lib.check(rc, 'dino_nodal_explicit_adjoint')
+    return gin, None, None
+
+
 # ------------------------------------------------------------------ equations
 
 
@@ -376,6 +428,9 @@ class PrimitiveEquationsSigma:
 
   implicit_inverse_method, vertical_matmul_method: accepted for API
   compatibility; every method gives the same operator here.
+  fused_nodal: True runs the nodal nonlinear terms as one fused HIP kernel
+  (with a hand-written adjoint); False runs them as separate tensor
+  operations, the reference's formulation.
   """
   reference_temperature: np.ndarray
   orography: Array
@@ -387,6 +442,7 @@ class PrimitiveEquationsSigma:
   include_vertical_advection: bool = dataclasses.field(default=True, kw_only=True)
   humidity_key: str | None = dataclasses.field(default=None, kw_only=True)
   cloud_keys: tuple | None = dataclasses.field(default=None, kw_only=True)
+  fused_nodal: bool = dataclasses.field(default=True, kw_only=True)
 
   def __post_init__(self):
     if not np.allclose(self.coords.horizontal.radius, self.physics_specs.radius, rtol=1e-5):
@@ -459,50 +515,90 @@ class PrimitiveEquationsSigma:
     return 2 * self.physics_specs.angular_velocity * sin_lat
 
   # ---- explicit terms
-  def explicit_terms(self, state: State) -> State:
-    """Explicit tendencies (primitive_equations.py:1126-1194)."""
-    validate_state_shape(state, self.coords)
+  def _nodal_handle(self, device, tracer_names):
+    key = (torch.device(device).index, tuple(tracer_names))
+    cache = self.__dict__.setdefault('_nodal_handles', {})
+    h = cache.get(key)
+    if h is not None:
+      return h
     grid = self.coords.horizontal
+    v = self.coords.vertical
     ps = self.physics_specs
+    K = v.layers
+    f64 = lambda a: np.ascontiguousarray(np.asarray(a, np.float64))
+    sec2 = f64(grid.sec2_lat)
+    cor = f64(2 * ps.angular_velocity * grid.nodal_axes[1])
+    dsig = f64(v.layer_thickness)
+    c2c = np.zeros(K)
+    c2c[:K - 1] = 1 / v.center_to_center
+    ss = np.zeros(K)
+    ss[:K - 1] = np.cumsum(v.layer_thickness)[:K - 1]
+    alpha = f64(get_sigma_ratios(v))
+    tref = f64(self.reference_temperature)
+    gw = get_geopotential_weights_sigma(v, ps.R)
+    geo_off = f64(np.concatenate([[0.0], gw[0, 1:]]))
+    geo_diag = f64(np.diag(gw))
+    hum = tracer_names.index(self.humidity_key) if self.humidity_key is not None else -1
+    clouds = [tracer_names.index(k) for k in (self.cloud_keys or ())]
+    cl = (ctypes.c_int * 4)(*(clouds + [0] * (4 - len(clouds))))
+    handle = ctypes.c_void_p()
+    dp = _lib.dptr
+    with torch.cuda.device(device):
+      rc = _lib.lib().dino_nodal_create(
+          K, grid.longitude_nodes, grid.latitude_nodes, len(tracer_names), hum, cl, len(clouds),
+          int(self.include_vertical_advection), dp(sec2), dp(cor), dp(dsig), dp(f64(c2c)),
+          dp(f64(ss)), dp(alpha), dp(tref), dp(geo_off), dp(geo_diag), float(ps.R),
+          float(ps.kappa), float(ps.R_vapor), float(ps.Cp), float(ps.Cp_vapor),
+          ctypes.byref(handle))
+    _lib.check(rc, 'dino_nodal_create')
+    h = _NodalHandle(handle)
+    cache[key] = h
+    return h
+
+  def _check_tracers(self, state):
+    names = list(state.tracers.keys())
+    if self.humidity_key is not None and self.humidity_key not in state.tracers:
+      raise ValueError(f'`{self.humidity_key}` is not found in tracers: {names}.')
+    for key in self.cloud_keys or ():
+      if key not in state.tracers:
+        raise ValueError(f'`{key}` is not found in tracers: {names}.')
+    return names
+
+  def _inverse_inputs(self, state, names):
+    """Modal fields for the one batched inverse transform, in the order of the
+    fused kernel's input groups (spherical_harmonic.py:1164-1178, clip=False)."""
+    grid = self.coords.horizontal
     a = grid.radius
     zeta, delta = state.vorticity, state.divergence
-    tv, lnps = state.temperature_variation, state.log_surface_pressure
-    x = zeta
-    moist = self.humidity_key is not None
-    tracer_names = list(state.tracers.keys())
-    if moist and self.humidity_key not in state.tracers:
-      raise ValueError(f'`{self.humidity_key}` is not found in tracers: {tracer_names}.')
-    if self.cloud_keys is not None:
-      for key in self.cloud_keys:
-        if key not in state.tracers:
-          raise ValueError(f'`{key}` is not found in tracers: {tracer_names}.')
-
-    # modal preparation (spherical_harmonic.py:1164-1178 with clip=False)
+    lnps = state.log_surface_pressure
     psi = grid.inverse_laplacian(zeta)
     chi = grid.inverse_laplacian(delta)
     ucos_m = (grid.d_dlon(chi) - grid.cos_lat_d_dlat(psi)) / a
     vcos_m = (grid.cos_lat_d_dlat(chi) + grid.d_dlon(psi)) / a
-    gx_m = grid.d_dlon(lnps) / a
-    gy_m = grid.cos_lat_d_dlat(lnps) / a
-    inv = [zeta, delta, tv] + [state.tracers[k] for k in tracer_names] + [ucos_m, vcos_m, gx_m, gy_m]
-    if moist:
+    inv = [zeta, delta, state.temperature_variation] + [state.tracers[k] for k in names]
+    inv += [ucos_m, vcos_m, grid.d_dlon(lnps) / a, grid.cos_lat_d_dlat(lnps) / a]
+    if self.humidity_key is not None:
       q_m = state.tracers[self.humidity_key]
       inv += [grid.laplacian(lnps), grid.d_dlon(q_m) / a, grid.cos_lat_d_dlat(q_m) / a]
-    nod = grid.to_nodal_batched(inv)
+    return inv
+
+  def _nodal_unfused(self, nod, names, x):
+    """The nodal nonlinear terms as separate tensor operations (the
+    reference's formulation, primitive_equations.py:168-225, 778-805,
+    861-1124); returns the forward-transform inputs in output-group order."""
+    ps = self.physics_specs
+    moist = self.humidity_key is not None
     zn, dn, tn = nod[0], nod[1], nod[2]
-    trn = dict(zip(tracer_names, nod[3:3 + len(tracer_names)]))
-    i0 = 3 + len(tracer_names)
+    trn = dict(zip(names, nod[3:3 + len(names)]))
+    i0 = 3 + len(names)
     un, vn, gxn, gyn = nod[i0:i0 + 4]
     if moist:
       lap_lnps_n, dqx_n, dqy_n = nod[i0 + 4:i0 + 7]
-
     sec2 = self._c('sec2', x)
     dsig = self._c('dsig', x)
     c2c = self._c('c2c_inv', x)
     tref = self._c('T_ref', x)
     alpha = self._c('alpha', x)
-
-    # diagnostic state (primitive_equations.py:168-225)
     udg = (un * gxn + vn * gyn) * sec2
     f_expl = torch.cumsum(udg * dsig, dim=-3)
     f_full = torch.cumsum((dn + udg) * dsig, dim=-3)
@@ -513,7 +609,6 @@ class PrimitiveEquationsSigma:
     def vadv(w, field):
       return _vadv(w, field, c2c, x)
 
-    # curl and divergence tendencies (:1001-1059)
     total_vort = zn + self._c('coriolis', x)
     nvu = -vn * total_vort * sec2
     nvv = un * total_vort * sec2
@@ -525,26 +620,17 @@ class PrimitiveEquationsSigma:
     if moist:
       q = trn[self.humidity_key]
       adj = 1 + (ps.R_vapor / ps.R - 1) * q
-      if self.cloud_keys is not None:
-        for key in self.cloud_keys:
-          adj = adj - trn[key]
+      for key in self.cloud_keys or ():
+        adj = adj - trn[key]
     else:
       adj = 1.0
     rt = ps.R * tn * adj
     comb_u = nvu + (sdu + rt * gxn) * sec2
     comb_v = nvv + (sdv + rt * gyn) * sec2
-    # kinetic energy (:778-785)
     ke = (un * un + vn * vn) * sec2 / 2
-
-    # temperature tendencies (:1061-1115)
-    if self.include_vertical_advection:
-      dT_v = vadv(sd_full, tn)
-    else:
-      dT_v = 0.0
+    dT_v = vadv(sd_full, tn) if self.include_vertical_advection else 0.0
     if np.unique(self.reference_temperature.ravel()).size > 1:
       dT_v = dT_v + vadv(sd_expl, tref)
-    g_expl = udg
-    g_full = udg + dn
 
     def t_omega(tfield, g_term):
       f = torch.cumsum(g_term * dsig, dim=-3)
@@ -552,34 +638,69 @@ class PrimitiveEquationsSigma:
       af_prev = torch.nn.functional.pad(af, (0, 0, 0, 0, 1, 0))[..., :-1, :, :]
       return tfield * (udg - (af + af_prev) / dsig)
 
-    mean_t = t_omega(tref, g_expl)
+    mean_t = t_omega(tref, udg)
     if not moist:
-      dT_a = ps.kappa * (mean_t + t_omega(tn, g_full))
+      dT_a = ps.kappa * (mean_t + t_omega(tn, udg + dn))
     else:
       gcr = ps.R_vapor / ps.R
       hcr = ps.Cp_vapor / ps.Cp
       vtc = tn * ((1 + (gcr - 1) * q) / (1 + (hcr - 1) * q))
       hrc = tref * (((gcr - hcr) * q) / (1 + (hcr - 1) * q))
-      dT_a = ps.kappa * (mean_t + t_omega(vtc + hrc, g_full))
-    # log surface pressure (:1117-1124)
+      dT_a = ps.kappa * (mean_t + t_omega(vtc + hrc, udg + dn))
     lsp_n = -(udg * dsig).sum(dim=-3, keepdim=True)
-    # horizontal advection of T and tracers (:795-805)
     temp_n = tn * dn + dT_v + dT_a
     fwd = [comb_u, comb_v, ke, temp_n, un * tn * sec2, vn * tn * sec2, lsp_n]
-    for k in tracer_names:
+    for k in names:
       s = trn[k]
       tr_v = vadv(sd_full, s) if self.include_vertical_advection else 0.0
       fwd += [tr_v + s * dn, un * s * sec2, vn * s * sec2]
     if moist:
       coef = tref * (ps.R_vapor - ps.R)
-      gx1, gy1 = gxn, gyn
-      curl_term = coef * sec2 * (gx1 * dqy_n - gy1 * dqx_n)
+      curl_term = coef * sec2 * (gxn * dqy_n - gyn * dqx_n)
       corr = q * lap_lnps_n * tref * (ps.R_vapor - ps.R)
-      dot_term = coef * sec2 * (dqx_n * gx1 + dqy_n * gy1)
+      dot_term = coef * sec2 * (dqx_n * gxn + dqy_n * gyn)
       tdiff = q * (tn + tref) * (ps.R_vapor / ps.R - 1)
       geo = torch.einsum('gh,...hij->...gij', self._c('geo_weights', x), tdiff)
       fwd += [curl_term, geo, dot_term + corr]
-    mod = grid.to_modal_batched(fwd)
+    return fwd
+
+  def _orography_modal(self, x):
+    key = ('orography', x.device, x.dtype)
+    t = self._consts.get(key)
+    if t is None:
+      oro = self.orography
+      t = (oro.to(x.device, x.dtype) if isinstance(oro, torch.Tensor)
+           else torch.as_tensor(np.asarray(oro), device=x.device).to(x.dtype))
+      self._consts[key] = t
+    return t
+
+  def explicit_terms(self, state: State) -> State:
+    """Explicit tendencies (primitive_equations.py:1126-1194)."""
+    validate_state_shape(state, self.coords)
+    names = self._check_tracers(state)
+    grid = self.coords.horizontal
+    sh = grid.spherical_harmonics
+    ps = self.physics_specs
+    a = grid.radius
+    x = state.vorticity
+    moist = self.humidity_key is not None
+    K = self.coords.vertical.layers
+    R_, L_ = grid.modal_shape
+    lead = tuple(x.shape[:-3])
+    members = int(np.prod(lead)) if lead else 1
+    inv = self._inverse_inputs(state, names)
+    if self.fused_nodal:
+      cat = torch.cat([t.reshape(-1, R_, L_) for t in inv], 0)
+      nodal = sh.inverse_transform(cat)
+      h = self._nodal_handle(x.device, names)
+      nout = _NodalFused.apply(nodal, h, members)
+      mcat = sh.transform(nout)
+      levels = [K] * 6 + [1] + [K] * (3 * len(names)) + ([K] * 3 if moist else [])
+      parts = torch.split(mcat, [members * kk for kk in levels], 0)
+      mod = [pp.reshape(lead + (kk, R_, L_)) for pp, kk in zip(parts, levels)]
+    else:
+      nod = grid.to_nodal_batched(inv)
+      mod = grid.to_modal_batched(self._nodal_unfused(nod, names, x))
     cu, cv, ke_m, temp_m, ut_m, vt_m, lsp_m = mod[:7]
 
     def div_cos(mm, nn):
@@ -590,16 +711,14 @@ class PrimitiveEquationsSigma:
 
     vort_t = -curl_cos(cu, cv)
     div_t = -div_cos(cu, cv) - grid.laplacian(ke_m)
-    oro = torch.as_tensor(np.asarray(self.orography), device=x.device).to(x.dtype) \
-        if not isinstance(self.orography, torch.Tensor) else self.orography.to(x.device, x.dtype)
-    div_t = div_t - ps.g * grid.laplacian(oro)
+    div_t = div_t - ps.g * grid.laplacian(self._orography_modal(x))
     temp_t = temp_m - div_cos(ut_m, vt_m)
     tracers_t = {}
-    for n, k in enumerate(tracer_names):
+    for n, k in enumerate(names):
       tm, um, vm = mod[7 + 3 * n: 10 + 3 * n]
       tracers_t[k] = tm - div_cos(um, vm)
     if moist:
-      j0 = 7 + 3 * len(tracer_names)
+      j0 = 7 + 3 * len(names)
       curl_m, geo_m, dc_m = mod[j0:j0 + 3]
       vort_t = vort_t + curl_m
       div_t = div_t + (-grid.laplacian(geo_m) - dc_m)

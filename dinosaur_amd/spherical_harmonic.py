@@ -754,9 +754,7 @@ def _batched(grid: Grid, leaves, inverse: bool):
     leads.append(t.shape[:-2])
     flat.append(t.reshape((-1,) + tail))
   cat = torch.cat(flat, 0) if len(flat) > 1 else flat[0]
-  plan = sh.plan(cat.device)
-  fn = _ToNodal if inverse else _ToModal
-  out = fn.apply(cat, plan, sh._precision)
+  out = sh.inverse_transform(cat) if inverse else sh.transform(cat)
   sizes = [f.shape[0] for f in flat]
   parts = torch.split(out, sizes, 0)
   return [p.reshape(lead + out_tail) for p, lead in zip(parts, leads)]

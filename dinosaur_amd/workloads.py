@@ -61,12 +61,13 @@ def humidity_nodal(grid, sigma_centers: np.ndarray, seed: int) -> np.ndarray:
 
 def build_primitive(name: str, layers: int, *, moist: bool = False, members: int = 1,
                     device='cuda', dtype=torch.float32, transform_precision='float32',
-                    held_suarez: bool = False):
+                    impl=None, fused_nodal: bool = True):
   """Returns (equation, state, coords, physics_specs) for the rest-atmosphere
   recipe with seeded orography and perturbations.  With members > 1 the state
-  leaves carry a leading member dimension."""
+  leaves carry a leading member dimension.  `impl` replaces the spherical
+  harmonics implementation (the Grid's plug-in seam)."""
   import functools  # pylint: disable=g-import-not-at-top
-  impl = functools.partial(spherical_harmonic.RealSphericalHarmonics,
+  impl = functools.partial(impl or spherical_harmonic.RealSphericalHarmonics,
                            transform_precision=transform_precision)
   grid = GRIDS[name](spherical_harmonics_impl=impl)
   vertical = sigma_coordinates.SigmaCoordinates.equidistant(layers)
@@ -114,5 +115,5 @@ def build_primitive(name: str, layers: int, *, moist: bool = False, members: int
               cloud_keys=('specific_cloud_liquid_water_content',
                           'specific_cloud_ice_water_content'))
   eq = primitive_equations.PrimitiveEquationsSigma(
-      aux['reference_temperature'], modal_oro, coords, ps, **kw)
+      aux['reference_temperature'], modal_oro, coords, ps, fused_nodal=fused_nodal, **kw)
   return eq, state, coords, ps

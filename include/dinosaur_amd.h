@@ -162,6 +162,45 @@ int dino_implicit_terms(const dino_implicit* f, const void* div,
                         void* temp_out, void* lnps_out, int64_t members,
                         int dtype, void* stream);
 
+/* Fused nodal nonlinear terms of PrimitiveEquationsSigma.explicit_terms.
+ * Replaces compute_diagnostic_state_sigma and the nodal parts of
+ * curl_and_div_tendencies, kinetic_energy_tendency,
+ * horizontal_scalar_advection, nodal_temperature_vertical_tendency,
+ * nodal_temperature_adiabatic_tendency, nodal_log_pressure_tendency and the
+ * humidity terms (dinosaur/primitive_equations.py:168-225, 778-805,
+ * 861-1124) with sigma_coordinates.centered_vertical_advection (:344-405).
+ *
+ * Buffers are (fields, I, J).  Input field groups, in order, each
+ * (members, levels): Z D T TRACER[0..n) U V GX GY [LAP_LNPS QX QY]
+ * (levels K except GX, GY, LAP_LNPS: 1; the bracketed groups only when
+ * humidity_index >= 0).  Output groups: CU CV KE TEMP UT VT LSP
+ * (TRN TU TV)[0..n) [CURL GEO DC].  Per-level constants are host FP64
+ * arrays of length K (c2c_inv and sum_sigma use the first K-1 entries);
+ * sec2 and coriolis have length J; geo_off[k] = G[0][k] (k >= 1, geo_off[0]
+ * unused) and geo_diag[k] = G[k][k] of the geopotential matrix (may be NULL
+ * for dry equations). */
+typedef struct dino_nodal dino_nodal;
+int dino_nodal_create(int K, int I, int J, int n_tracers, int humidity_index,
+                      const int* cloud_indices, int n_clouds,
+                      int include_vertical_advection, const double* sec2,
+                      const double* coriolis, const double* dsigma,
+                      const double* c2c_inv, const double* sum_sigma,
+                      const double* alpha, const double* tref,
+                      const double* geo_off, const double* geo_diag,
+                      double r_gas, double kappa, double r_vapor, double cp,
+                      double cp_vapor, dino_nodal** out);
+int dino_nodal_destroy(dino_nodal* n);
+/* Number of input (output = 0) or output (output = 1) fields for `members`. */
+int64_t dino_nodal_fields(const dino_nodal* n, int64_t members, int output);
+int dino_nodal_explicit(const dino_nodal* n, const void* nodal_in,
+                        void* nodal_out, int64_t members, int dtype,
+                        void* stream);
+/* Vector-Jacobian product: input cotangents from output cotangents (the
+ * reference obtains this by XLA autodiff of the same elementwise graph). */
+int dino_nodal_explicit_adjoint(const dino_nodal* n, const void* nodal_in,
+                                const void* out_cot, void* in_cot,
+                                int64_t members, int dtype, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -246,27 +246,33 @@ class Grid:
     p = np.repeat(p, 2, axis=0)[1:]
     return f, p, w
 
+  def _basis_as(self, dtype):
+    """FP64 tables cast to the data dtype, as the reference's einsum does."""
+    cache = self.__dict__.setdefault('_cast_cache', {})
+    key = np.dtype(dtype)
+    if key not in cache:
+      cache[key] = tuple(a.astype(key) for a in self.basis)
+    return cache[key]
+
   # ---- transforms (spherical_harmonic.py:262-280, 1017-1031)
   def to_nodal(self, x):
     if isinstance(x, dict):
       return {k: self.to_nodal(v) for k, v in x.items()}
     if isinstance(x, (tuple, list)):
       return type(x)(self.to_nodal(v) for v in x)
-    f, p, _ = self.basis
-    dt = x.dtype
-    px = np.einsum('mjl,...ml->...mj', p.astype(dt), x)
-    return np.einsum('im,...mj->...ij', f.astype(dt), px)
+    f, p, _ = self._basis_as(x.dtype)
+    px = np.einsum('mjl,...ml->...mj', p, x, optimize=True)
+    return np.einsum('im,...mj->...ij', f, px, optimize=True)
 
   def to_modal(self, z):
     if isinstance(z, dict):
       return {k: self.to_modal(v) for k, v in z.items()}
     if isinstance(z, (tuple, list)):
       return type(z)(self.to_modal(v) for v in z)
-    f, p, w = self.basis
-    dt = z.dtype
-    wz = w.astype(dt) * z
-    fwz = np.einsum('im,...ij->...mj', f.astype(dt), wz)
-    return np.einsum('mjl,...mj->...ml', p.astype(dt), fwz)
+    f, p, w = self._basis_as(z.dtype)
+    wz = w * z
+    fwz = np.einsum('im,...ij->...mj', f, wz, optimize=True)
+    return np.einsum('mjl,...mj->...ml', p, fwz, optimize=True)
 
   # ---- spectral operators (spherical_harmonic.py:1033-1140)
   def laplacian(self, x):

@@ -12,6 +12,8 @@
 //
 // Real-basis conventions (fourier.py:24-60): row 0 is 1/sqrt(2 pi), row 2m-1
 // is cos(m lambda)/sqrt(pi), row 2m is sin(m lambda)/sqrt(pi).
+#include <type_traits>
+
 #include "dino_common.h"
 
 namespace dino {
@@ -147,6 +149,93 @@ __device__ __forceinline__ void fft_lds(cplx<T>* buf, const cplx<T>* tws) {
   }
 }
 
+// Tile (N rows x C columns starting at column j0) <-> LDS [n][c].  When the
+// tile is fully inside J and 16-byte aligned, every thread issues all of its
+// vector loads before any LDS store (compile-time trip count), so a
+// workgroup keeps its whole tile in flight; otherwise a guarded scalar loop.
+template <typename T, int N, int C, int NT>
+__device__ __forceinline__ void load_tile(const T* __restrict__ src, int J, int j0,
+                                          const T* __restrict__ qw, T* lds) {
+  constexpr int V = 16 / sizeof(T);  // elements per 16-byte vector
+  const int tid = threadIdx.x;
+  if constexpr (C % V == 0 && (N * C / V) % NT == 0) {
+    if (j0 + C <= J && (J % V) == 0) {
+      using VT = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+      constexpr int CV = C / V;
+      constexpr int IT = N * CV / NT;
+      static_assert(NT % CV == 0, "column vector index must be fixed per thread");
+      VT v[IT];
+      const int cv = tid % CV;  // fixed across iterations since NT % CV == 0
+#pragma unroll
+      for (int q = 0; q < IT; ++q) {
+        int idx = tid + q * NT;
+        int n = idx / CV;
+        v[q] = *reinterpret_cast<const VT*>(src + (int64_t)n * J + j0 + cv * V);
+      }
+      T w[V];
+#pragma unroll
+      for (int e = 0; e < V; ++e) w[e] = qw ? qw[j0 + cv * V + e] : (T)1;
+#pragma unroll
+      for (int q = 0; q < IT; ++q) {
+        int idx = tid + q * NT;
+        int n = idx / CV;
+        const T* pv = reinterpret_cast<const T*>(&v[q]);
+#pragma unroll
+        for (int e = 0; e < V; ++e) lds[n * C + cv * V + e] = pv[e] * w[e];
+      }
+      return;
+    }
+  }
+  for (int idx = tid; idx < N * C; idx += NT) {
+    int n = idx / C, c = idx % C;
+    int j = j0 + c;
+    T v = 0;
+    if (j < J) {
+      v = src[(int64_t)n * J + j];
+      if (qw) v *= qw[j];
+    }
+    lds[n * C + c] = v;
+  }
+}
+
+template <typename T, int N, int C, int NT>
+__device__ __forceinline__ void store_tile(const T* lds, T* __restrict__ dst, int J, int j0,
+                                           const T* __restrict__ qw) {
+  constexpr int V = 16 / sizeof(T);
+  const int tid = threadIdx.x;
+  if constexpr (C % V == 0 && (N * C / V) % NT == 0) {
+    if (j0 + C <= J && (J % V) == 0) {
+      using VT = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+      constexpr int CV = C / V;
+      constexpr int IT = N * CV / NT;
+      static_assert(NT % CV == 0, "column vector index must be fixed per thread");
+      const int cv = tid % CV;
+      T w[V];
+#pragma unroll
+      for (int e = 0; e < V; ++e) w[e] = qw ? qw[j0 + cv * V + e] : (T)1;
+#pragma unroll
+      for (int q = 0; q < IT; ++q) {
+        int idx = tid + q * NT;
+        int n = idx / CV;
+        VT v;
+        T* pv = reinterpret_cast<T*>(&v);
+#pragma unroll
+        for (int e = 0; e < V; ++e) pv[e] = lds[n * C + cv * V + e] * w[e];
+        *reinterpret_cast<VT*>(dst + (int64_t)n * J + j0 + cv * V) = v;
+      }
+      return;
+    }
+  }
+  for (int idx = tid; idx < N * C; idx += NT) {
+    int n = idx / C, c = idx % C;
+    int j = j0 + c;
+    if (j >= J) continue;
+    T v = lds[n * C + c];
+    if (qw) v *= qw[j];
+    dst[(int64_t)n * J + j] = v;
+  }
+}
+
 // nodal (B, N, J) -> fourier (B, R, J)
 template <typename T, int N, int P, int NT>
 __global__ __launch_bounds__(NT) void dft_forward_fft_kernel(
@@ -161,16 +250,8 @@ __global__ __launch_bounds__(NT) void dft_forward_fft_kernel(
   for (int k = tid; k < N; k += NT) tws[k] = cplx<T>{tw[2 * k], -tw[2 * k + 1]};
   const T* src = nodal + b * (int64_t)N * J;
   T* bufr = reinterpret_cast<T*>(buf);
-  for (int idx = tid; idx < N * 2 * P; idx += NT) {
-    int n = idx / (2 * P), c = idx % (2 * P);
-    int j = j0 + c;
-    T v = 0;
-    if (j < J) {
-      v = src[(int64_t)n * J + j];
-      if (weighted) v *= qw[j];
-    }
-    bufr[n * 2 * P + c] = v;  // (re, im) of pair c/2 interleave as x_a, x_b
-  }
+  // (re, im) of pair c/2 interleave as x_a, x_b
+  load_tile<T, N, 2 * P, NT>(src, J, j0, weighted ? qw : nullptr, bufr);
   __syncthreads();
   fft_lds<T, N, P, NT, false>(buf, tws);
   const T inv_sqrt_2pi = (T)0.39894228040143267794;
@@ -236,15 +317,8 @@ __global__ __launch_bounds__(NT) void dft_inverse_fft_kernel(
   __syncthreads();
   fft_lds<T, N, P, NT, true>(buf, tws);
   T* dst = nodal + b * (int64_t)N * J;
-  const T* bufr = reinterpret_cast<const T*>(buf);
-  for (int idx = tid; idx < N * 2 * P; idx += NT) {
-    int n = idx / (2 * P), c = idx % (2 * P);
-    int j = j0 + c;
-    if (j >= J) continue;
-    T v = bufr[n * 2 * P + c];
-    if (weighted) v *= qw[j];
-    dst[(int64_t)n * J + j] = v;
-  }
+  store_tile<T, N, 2 * P, NT>(reinterpret_cast<const T*>(buf), dst, J, j0,
+                              weighted ? qw : nullptr);
 }
 
 // ---------------------------------------------------------------- direct DFT

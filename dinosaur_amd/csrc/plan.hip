@@ -188,7 +188,7 @@ using namespace dino;
 
 extern "C" const char* dino_last_error(void) { return g_last_error.c_str(); }
 
-extern "C" const char* dino_version(void) { return "dinosaur_amd 0.1 gfx950"; }
+extern "C" const char* dino_version(void) { return "dinosaur_amd 0.2 gfx950 mfma-f32 split-bf16x3"; }
 
 extern "C" int dino_gauss_nodes_host(int n, double* nodes, double* weights) {
   DINO_REQUIRE(n >= 1 && nodes && weights, DINO_EINVAL,

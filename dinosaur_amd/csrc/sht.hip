@@ -30,15 +30,16 @@ int check_common(const dino_plan* p, const void* a, const void* b, int64_t batch
   return DINO_OK;
 }
 
+// FP64 data: FP64 FMA tiles.  FP32 data: MFMA (FP32 or split BF16 x3).
 int leg_inverse(const dino_plan* p, const void* in, void* out, int64_t B, int dtype, int prec,
                 hipStream_t s) {
-  if (prec == DINO_PREC_NATIVE) return legendre_inverse_fma(p, in, out, B, dtype, s);
+  if (dtype == DINO_F64) return legendre_inverse_fma(p, in, out, B, dtype, s);
   return legendre_inverse_mfma(p, in, out, B, prec, s);
 }
 
 int leg_forward(const dino_plan* p, const void* in, void* out, int64_t B, int dtype, int prec,
                 hipStream_t s) {
-  if (prec == DINO_PREC_NATIVE) return legendre_forward_fma(p, in, out, B, dtype, s);
+  if (dtype == DINO_F64) return legendre_forward_fma(p, in, out, B, dtype, s);
   return legendre_forward_mfma(p, in, out, B, prec, s);
 }
 

@@ -132,3 +132,26 @@ def test_grid_operators_match_oracle(gpu_device):
   assert _rel(u.cpu().numpy(), ou) < 1e-14 and _rel(v.cpu().numpy(), ov) < 1e-14
   d = g.div_cos_lat((xt[0], xt[1])).cpu().numpy()
   assert _rel(d, og.div_cos_lat((x[0], x[1]))) < 1e-14
+
+
+@pytest.mark.parametrize('name', ['T21', 'TL127'])
+def test_split_bf16x3_transforms_within_1e5(name, gpu_device):
+  """Precision pattern: split-BF16 x3 MFMA Legendre with FP32 accumulation
+  stays within 1e-5 normwise (max-abs and RMS) of the FP64 oracle."""
+  import functools
+  impl = functools.partial(sh.RealSphericalHarmonics, transform_precision='bf16x3')
+  g = getattr(sh.Grid, name)(spherical_harmonics_impl=impl)
+  og = getattr(osht.Grid, name)()
+  x = _modal(og, 6, 9)
+  for i, s in enumerate([1.0, 1e-2, 300.0, 1e-5, 1.0, 1.0]):
+    x[i] *= s
+  y = g.to_nodal(torch.tensor(x, dtype=torch.float32, device=gpu_device)).cpu().numpy()
+  yr = og.to_nodal(x)
+  z = g.to_modal(torch.tensor(yr, dtype=torch.float32, device=gpu_device)).cpu().numpy()
+  zr = og.to_modal(yr)
+  for i in range(6):
+    for got, ref, what in ((y[i], yr[i], 'to_nodal'), (z[i], zr[i], 'to_modal')):
+      mx = _rel(got, ref)
+      rms = np.sqrt(((got - ref) ** 2).mean() / (ref ** 2).mean())
+      print(f'bf16x3 {name} field {i} {what}: max-rel {mx:.2e} rms-rel {rms:.2e}', flush=True)
+      assert mx < 1e-5 and rms < 1e-5

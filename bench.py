@@ -47,7 +47,7 @@ def _parse():
   ap.add_argument('--steps', type=int, default=4)
   ap.add_argument('--warmup', type=int, default=1)
   ap.add_argument('--members', type=int, default=MEMBERS)
-  ap.add_argument('--precision', default='auto', help='float32 | f16x3 | bf16x3 | auto')
+  ap.add_argument('--precision', default='auto', help='float32 | bf16x3 | auto (bf16x3 when the library has split tables)')
   ap.add_argument('--plain-steps', type=int, default=1)
   ap.add_argument('--no-plain', action='store_true')
   ap.add_argument('--no-cpu', action='store_true')
@@ -258,7 +258,7 @@ def main():
   torch.cuda.set_device(dev)
   precision = args.precision
   if precision == 'auto':
-    precision = 'f16x3' if _lib.lib() and _split_available(torch) else 'float32'
+    precision = 'bf16x3' if _lib.lib() and _split_available(torch) else 'float32'
   members = args.members
   print(f'[bench] chosen leg: {GRID} L{LAYERS} moist+3 tracers, {members} members, '
         f'precision={precision}', file=sys.stderr, flush=True)

@@ -58,6 +58,14 @@ SIGNATURES = {
     'dino_nodal_fields': (_c_i64, [_c_p, _c_i64, _c_i]),
     'dino_nodal_explicit': (_c_i, [_c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
     'dino_nodal_explicit_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
+    'dino_modal_create': (_c_i, [_c_i, _c_i, _c_i, _c_i, _c_i, _c_d, _c_d, _c_dp, _c_dp, _c_dp,
+                                 _c_dp, _c_dp, _c_dp, _c_dp, ctypes.POINTER(_c_p)]),
+    'dino_modal_destroy': (_c_i, [_c_p]),
+    'dino_modal_fields': (_c_i64, [_c_p, _c_i64, _c_i]),
+    'dino_modal_prep': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
+    'dino_modal_prep_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
+    'dino_modal_epilogue': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
+    'dino_modal_epilogue_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
 }
 
 _lib = None

@@ -92,15 +92,17 @@ struct PolicySplit {
 };
 
 // ------------------------------------------------------------ inverse
-// KH degrees per parity per chunk (chunk = 2 KH consecutive degrees).
+// KH degrees per parity per chunk (chunk = 2 KH consecutive degrees).  The
+// next chunk is fetched into registers while the current one is multiplied.
 template <class Pol, int TC, int KH>
 __global__ __launch_bounds__(kThreads) void leg_inv_mfma_kernel(
     const float* __restrict__ modal, float* __restrict__ fourier, const float* __restrict__ ptab,
-    const int64_t* __restrict__ m_off, const float* __restrict__ scale, int B, int L, int J,
-    int Jh, int Jhp, int Rrows) {
+    const int64_t* __restrict__ m_off, int B, int L, int J, int Jh, int Jhp, int Rrows) {
   constexpr int WR = TR / 2, WC = TC / 2, MT = WR / 16, NT = WC / 16;
   constexpr int KC = 2 * KH;
   constexpr int PA = TR + 4, PB = TC + 4;
+  constexpr int A_PER = TR * KC / kThreads, P_PER = KC * TC / kThreads;
+  static_assert(TR * KC % kThreads == 0 && KC * TC % kThreads == 0, "tile / thread mismatch");
   const int m = blockIdx.z;
   const int nrows = (m == 0) ? B : 2 * B;
   const int row0 = blockIdx.y * TR;
@@ -116,28 +118,52 @@ __global__ __launch_bounds__(kThreads) void leg_inv_mfma_kernel(
 #pragma unroll
     for (int c = 0; c < NT; ++c) accE[a][c] = accO[a][c] = floatx4{0, 0, 0, 0};
   const float* pm = ptab + m_off[m];
-  for (int l0 = m; l0 < L; l0 += KC) {
-    for (int idx = tid; idx < TR * KC; idx += kThreads) {
-      int q = idx / KC, k = idx % KC;
-      int row = row0 + q, l = l0 + k;
-      float v = 0.f;
-      if (row < nrows && l < L) {
-        int b, r;
-        row_of(m, row, &b, &r);
-        v = modal[((int64_t)b * Rrows + r) * L + l];
-        if (scale) v *= scale[(int64_t)b * Rrows + r];
-      }
-      if (k & 1) Ao[k >> 1][q] = v;
-      else Ae[k >> 1][q] = v;
+  // The rows a thread loads are q0 + i * (kThreads / KC): a fixed stride of
+  // rows, i.e. (m == 0 ? 1 : 1/2) * stride fields in memory.
+  constexpr int QS = kThreads / KC;
+  static_assert(kThreads % KC == 0 && QS % 2 == 0, "row stride must be even");
+  const int q0 = tid / KC, kq = tid % KC;
+  const float* abase = nullptr;
+  {
+    int b, r;
+    row_of(m, row0 + q0, &b, &r);
+    abase = modal + ((int64_t)b * Rrows + r) * L;
+  }
+  const int64_t astride = (int64_t)(m == 0 ? QS : QS / 2) * Rrows * L;
+  float ra[A_PER], rp[P_PER];
+  auto fetch = [&](int l0) {
+    const int l = l0 + kq;
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      bool ok = (row0 + q0 + i * QS < nrows) && l < L;
+      ra[i] = ok ? abase[i * astride + l] : 0.f;
     }
-    for (int idx = tid; idx < KC * TC; idx += kThreads) {
+#pragma unroll
+    for (int i = 0; i < P_PER; ++i) {
+      int idx = tid + i * kThreads;
       int k = idx / TC, c = idx % TC;
       int l = l0 + k, jj = jj0 + c;
-      float v = (l < L && jj < Jhp) ? pm[(int64_t)(l - m) * Jhp + jj] : 0.f;
-      if (k & 1) Po[k >> 1][c] = v;
-      else Pe[k >> 1][c] = v;
+      rp[i] = (l < L && jj < Jhp) ? pm[(int64_t)(l - m) * Jhp + jj] : 0.f;
+    }
+  };
+  fetch(m);
+  for (int l0 = m; l0 < L; l0 += KC) {
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      int idx = tid + i * kThreads;
+      int q = idx / KC, k = idx % KC;
+      if (k & 1) Ao[k >> 1][q] = ra[i];
+      else Ae[k >> 1][q] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < P_PER; ++i) {
+      int idx = tid + i * kThreads;
+      int k = idx / TC, c = idx % TC;
+      if (k & 1) Po[k >> 1][c] = rp[i];
+      else Pe[k >> 1][c] = rp[i];
     }
     __syncthreads();
+    if (l0 + KC < L) fetch(l0 + KC);
 #pragma unroll
     for (int ks = 0; ks < KH; ks += Pol::KSTEP) {
 #pragma unroll
@@ -160,7 +186,6 @@ __global__ __launch_bounds__(kThreads) void leg_inv_mfma_kernel(
       if (row >= nrows) continue;
       int b, r;
       row_of(m, row, &b, &r);
-      float inv = scale ? 1.f / scale[(int64_t)b * Rrows + r] : 1.f;
       float* out = fourier + ((int64_t)b * Rrows + r) * J;
 #pragma unroll
       for (int c = 0; c < NT; ++c) {
@@ -168,42 +193,46 @@ __global__ __launch_bounds__(kThreads) void leg_inv_mfma_kernel(
         if (jj >= Jh) continue;
         int jup = J - Jh + jj, jdn = Jh - 1 - jj;
         float e = accE[a][c][i], o = accO[a][c][i];
-        out[jup] = (e + o) * inv;
-        if (jdn != jup) out[jdn] = (e - o) * inv;
+        out[jup] = e + o;
+        if (jdn != jup) out[jdn] = e - o;
       }
     }
   }
 }
 
 // ------------------------------------------------------------ forward
-// KC latitudes per chunk; tile of 128 degrees starting at l = m + 128 * tile.
+// Degree tiles of 128 cover [0, L).  Within a tile, even columns c pair with
+// S and odd with A when (lt0 - m) is even (roles swap otherwise); P is zero
+// for l < m, and tiles entirely below the diagonal only write zeros.
 template <class Pol, int KC>
 __global__ __launch_bounds__(kThreads) void leg_fwd_mfma_kernel(
     const float* __restrict__ fourier, float* __restrict__ modal, const float* __restrict__ ptab,
-    const int64_t* __restrict__ m_off, const float* __restrict__ scale, int B, int L, int J,
-    int Jh, int Jhp, int Rrows) {
+    const int64_t* __restrict__ m_off, int B, int L, int J, int Jh, int Jhp, int Rrows) {
   constexpr int TLH = 64;  // degrees per parity in the tile
   constexpr int WR = TR / 2, WC = TLH / 2, MT = WR / 16, NT = WC / 16;
   constexpr int PA = TR + 4, PB = TLH + 4;
+  constexpr int S_PER = TR * KC / kThreads, P_PER = 2 * TLH * KC / kThreads;
+  static_assert(TR * KC % kThreads == 0 && 2 * TLH * KC % kThreads == 0, "tile mismatch");
   const int m = blockIdx.z;
   const int nrows = (m == 0) ? B : 2 * B;
   const int row0 = blockIdx.y * TR;
   if (row0 >= nrows) return;
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-  const int lt0 = m + blockIdx.x * (2 * TLH);
-  if (blockIdx.x == 0 && m > 0) {
-    // structural zeros for l < m
-    for (int idx = tid; idx < TR * m; idx += kThreads) {
-      int q = idx / m, l = idx % m;
-      int row = row0 + q;
-      if (row < nrows) {
+  const int lt0 = blockIdx.x * (2 * TLH);
+  if (lt0 >= L) return;
+  if (lt0 + 2 * TLH <= m) {
+    for (int idx = tid; idx < TR * 2 * TLH; idx += kThreads) {
+      int q = idx / (2 * TLH), c = idx % (2 * TLH);
+      int row = row0 + q, l = lt0 + c;
+      if (row < nrows && l < L) {
         int b, r;
         row_of(m, row, &b, &r);
         modal[((int64_t)b * Rrows + r) * L + l] = 0.f;
       }
     }
+    return;
   }
-  if (lt0 >= L) return;
+  const bool flip = ((lt0 - m) & 1) != 0;
   __shared__ float Ss[KC][PA], Aa[KC][PA];
   __shared__ float Pe[KC][PB], Po[KC][PB];
   const int wr = wave / 2, wc = wave % 2;
@@ -214,41 +243,54 @@ __global__ __launch_bounds__(kThreads) void leg_fwd_mfma_kernel(
     for (int c = 0; c < NT; ++c) accE[a][c] = accO[a][c] = floatx4{0, 0, 0, 0};
   const float* pm = ptab + m_off[m];
   const bool odd_eq = (J & 1);
-  for (int jj0 = 0; jj0 < Jh; jj0 += KC) {
-    for (int idx = tid; idx < TR * KC; idx += kThreads) {
-      int q = idx / KC, k = idx % KC;
-      int row = row0 + q, jj = jj0 + k;
-      float sv = 0.f, av = 0.f;
-      if (row < nrows && jj < Jh) {
-        int b, r;
-        row_of(m, row, &b, &r);
-        const float* f = fourier + ((int64_t)b * Rrows + r) * J;
-        int jup = J - Jh + jj, jdn = Jh - 1 - jj;
-        float u = f[jup];
-        if (odd_eq && jj == 0) {
-          sv = u;
-        } else {
-          float d = f[jdn];
-          sv = u + d;
-          av = u - d;
-        }
-        if (scale) {
-          float s = scale[(int64_t)b * Rrows + r];
-          sv *= s;
-          av *= s;
-        }
-      }
-      Ss[k][q] = sv;
-      Aa[k][q] = av;
+  constexpr int QS = kThreads / KC;
+  static_assert(kThreads % KC == 0 && QS % 2 == 0, "row stride must be even");
+  const int q0 = tid / KC, kq = tid % KC;
+  const float* fbase = nullptr;
+  {
+    int b, r;
+    row_of(m, row0 + q0, &b, &r);
+    fbase = fourier + ((int64_t)b * Rrows + r) * J;
+  }
+  const int64_t fstride = (int64_t)(m == 0 ? QS : QS / 2) * Rrows * J;
+  float ru[S_PER], rd[S_PER], rp[P_PER];
+  auto fetch = [&](int jj0) {
+    const int jj = jj0 + kq;
+    const bool eq = odd_eq && jj == 0;
+#pragma unroll
+    for (int i = 0; i < S_PER; ++i) {
+      bool ok = (row0 + q0 + i * QS < nrows) && jj < Jh;
+      ru[i] = ok ? fbase[i * fstride + J - Jh + jj] : 0.f;
+      rd[i] = (ok && !eq) ? fbase[i * fstride + Jh - 1 - jj] : 0.f;
     }
-    for (int idx = tid; idx < 2 * TLH * KC; idx += kThreads) {
+#pragma unroll
+    for (int i = 0; i < P_PER; ++i) {
+      int idx = tid + i * kThreads;
       int c = idx / KC, k = idx % KC;
       int l = lt0 + c, jj = jj0 + k;
-      float v = (l < L && jj < Jhp) ? pm[(int64_t)(l - m) * Jhp + jj] : 0.f;
-      if (c & 1) Po[k][c >> 1] = v;
-      else Pe[k][c >> 1] = v;
+      rp[i] = (l >= m && l < L && jj < Jhp) ? pm[(int64_t)(l - m) * Jhp + jj] : 0.f;
+    }
+  };
+  fetch(0);
+  for (int jj0 = 0; jj0 < Jh; jj0 += KC) {
+#pragma unroll
+    for (int i = 0; i < S_PER; ++i) {
+      int idx = tid + i * kThreads;
+      int q = idx / KC, k = idx % KC;
+      Ss[k][q] = ru[i] + rd[i];
+      Aa[k][q] = ru[i] - rd[i];
+    }
+#pragma unroll
+    for (int i = 0; i < P_PER; ++i) {
+      int idx = tid + i * kThreads;
+      int c = idx / KC, k = idx % KC;
+      if (c & 1) Po[k][c >> 1] = rp[i];
+      else Pe[k][c >> 1] = rp[i];
     }
     __syncthreads();
+    if (jj0 + KC < Jh) fetch(jj0 + KC);
+    const float(*Xe)[PA] = flip ? Aa : Ss;
+    const float(*Xo)[PA] = flip ? Ss : Aa;
 #pragma unroll
     for (int ks = 0; ks < KC; ks += Pol::KSTEP) {
 #pragma unroll
@@ -256,8 +298,8 @@ __global__ __launch_bounds__(kThreads) void leg_fwd_mfma_kernel(
 #pragma unroll
         for (int c = 0; c < NT; ++c) {
           int ar = wr * WR + a * 16, bc = wc * WC + c * 16;
-          accE[a][c] = Pol::template mma<PA, PB>(Ss, ar, Pe, bc, ks, lane, accE[a][c]);
-          accO[a][c] = Pol::template mma<PA, PB>(Aa, ar, Po, bc, ks, lane, accO[a][c]);
+          accE[a][c] = Pol::template mma<PA, PB>(Xe, ar, Pe, bc, ks, lane, accE[a][c]);
+          accO[a][c] = Pol::template mma<PA, PB>(Xo, ar, Po, bc, ks, lane, accO[a][c]);
         }
       }
     }
@@ -271,25 +313,27 @@ __global__ __launch_bounds__(kThreads) void leg_fwd_mfma_kernel(
       if (row >= nrows) continue;
       int b, r;
       row_of(m, row, &b, &r);
-      float inv = scale ? 1.f / scale[(int64_t)b * Rrows + r] : 1.f;
       float* out = modal + ((int64_t)b * Rrows + r) * L;
 #pragma unroll
       for (int c = 0; c < NT; ++c) {
         int ce = wc * WC + c * 16 + lane % 16;
         int le = lt0 + 2 * ce, lo = le + 1;
-        if (le < L) out[le] = accE[a][c][i] * inv;
-        if (lo < L) out[lo] = accO[a][c][i] * inv;
+        if (le < L) out[le] = accE[a][c][i];
+        if (lo < L) out[lo] = accO[a][c][i];
       }
     }
   }
 }
 
+}  // namespace
+
+namespace {
+
 template <class Pol, int TC, int KH>
 int launch_inv(const dino_plan* pl, const float* modal, float* fourier, int64_t B, hipStream_t s) {
   dim3 grid((pl->Jh + TC - 1) / TC, (unsigned)((2 * B + TR - 1) / TR), pl->M);
   leg_inv_mfma_kernel<Pol, TC, KH><<<grid, kThreads, 0, s>>>(
-      modal, fourier, pl->p32, pl->m_offset, nullptr, (int)B, pl->L, pl->J, pl->Jh, pl->Jhp,
-      pl->R);
+      modal, fourier, pl->p32, pl->m_offset, (int)B, pl->L, pl->J, pl->Jh, pl->Jhp, pl->R);
   DINO_LAUNCH_CHECK("legendre inverse mfma kernel");
   return DINO_OK;
 }
@@ -298,15 +342,14 @@ template <class Pol, int KC>
 int launch_fwd(const dino_plan* pl, const float* fourier, float* modal, int64_t B, hipStream_t s) {
   dim3 grid((pl->L + 127) / 128, (unsigned)((2 * B + TR - 1) / TR), pl->M);
   leg_fwd_mfma_kernel<Pol, KC><<<grid, kThreads, 0, s>>>(
-      fourier, modal, pl->p32, pl->m_offset, nullptr, (int)B, pl->L, pl->J, pl->Jh, pl->Jhp,
-      pl->R);
+      fourier, modal, pl->p32, pl->m_offset, (int)B, pl->L, pl->J, pl->Jh, pl->Jhp, pl->R);
   DINO_LAUNCH_CHECK("legendre forward mfma kernel");
   return DINO_OK;
 }
 
 template <class Pol, int KH>
 int inv_by_width(const dino_plan* pl, const float* in, float* out, int64_t B, hipStream_t s) {
-  if (pl->Jh > 64) return launch_inv<Pol, (KH > 8 ? 64 : 128), KH>(pl, in, out, B, s);
+  if (pl->Jh > 64) return launch_inv<Pol, (KH > 16 ? 64 : 128), KH>(pl, in, out, B, s);
   if (pl->Jh > 32) return launch_inv<Pol, 64, KH>(pl, in, out, B, s);
   return launch_inv<Pol, 32, KH>(pl, in, out, B, s);
 }
@@ -320,7 +363,7 @@ int legendre_inverse_mfma(const dino_plan* pl, const void* modal, void* fourier,
   const float* in = static_cast<const float*>(modal);
   float* out = static_cast<float*>(fourier);
   switch (precision) {
-    case DINO_PREC_NATIVE: return inv_by_width<PolicyF32, 8>(pl, in, out, B, s);
+    case DINO_PREC_NATIVE: return inv_by_width<PolicyF32, 16>(pl, in, out, B, s);
     case DINO_PREC_BF16X3: return inv_by_width<PolicySplit<true>, 32>(pl, in, out, B, s);
     default:
       set_error("split precision %d is not available; use DINO_PREC_BF16X3", precision);
@@ -335,7 +378,7 @@ int legendre_forward_mfma(const dino_plan* pl, const void* fourier, void* modal,
   const float* in = static_cast<const float*>(fourier);
   float* out = static_cast<float*>(modal);
   switch (precision) {
-    case DINO_PREC_NATIVE: return launch_fwd<PolicyF32, 16>(pl, in, out, B, s);
+    case DINO_PREC_NATIVE: return launch_fwd<PolicyF32, 32>(pl, in, out, B, s);
     case DINO_PREC_BF16X3: return launch_fwd<PolicySplit<true>, 32>(pl, in, out, B, s);
     default:
       set_error("split precision %d is not available; use DINO_PREC_BF16X3", precision);

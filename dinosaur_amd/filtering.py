@@ -28,7 +28,16 @@ def _make_filter_fn(scaling: np.ndarray):
       return x * s
     return (x * scaling).astype(x.dtype)
 
-  return lambda tree: spherical_harmonic.tree_map_over_nonscalars(rescale, tree)
+  def apply(tree):
+    flat = getattr(tree, 'flat_buffer', None)
+    buf = flat() if flat is not None else None
+    if buf is not None and buf.shape[-1] == scaling.shape[-1] and scaling.ndim == 1:
+      # every leaf of a flat-backed state ends in the (.., L) axis the
+      # scaling broadcasts over: one operation on the whole buffer
+      return type(tree).from_flat(rescale(buf), *tree._flat[1:], sim_time=tree.sim_time)
+    return spherical_harmonic.tree_map_over_nonscalars(rescale, tree)
+
+  return apply
 
 
 def exponential_filter(grid, attenuation=16, order=18, cutoff=0):

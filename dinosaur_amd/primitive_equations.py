@@ -46,15 +46,54 @@ Array = Any
 
 @dataclasses.dataclass
 class State:
-  """Spectral state of the primitive equations (primitive_equations.py:61-84)."""
+  """Spectral state of the primitive equations (primitive_equations.py:61-84).
+
+  A state may be backed by one contiguous (fields, 2M-1, L) buffer whose
+  views are the leaves, in the order vorticity, divergence,
+  temperature_variation, log_surface_pressure, tracers (each member-major).
+  Arithmetic on two such states is one tensor operation on the buffers;
+  states built leaf by leaf work the same way, one operation per leaf.
+  """
   vorticity: Array
   divergence: Array
   temperature_variation: Array
   log_surface_pressure: Array
   tracers: Mapping[str, Array] = dataclasses.field(default_factory=dict)
   sim_time: float | None = None
+  _flat: Any = dataclasses.field(default=None, repr=False, compare=False)
 
   _ARRAYS = ('vorticity', 'divergence', 'temperature_variation', 'log_surface_pressure')
+
+  @classmethod
+  def from_flat(cls, buf, lead, K, names, sim_time=None):
+    """State whose leaves are views of buf (fields, R, L)."""
+    E = int(np.prod(lead)) if lead else 1
+    tail = tuple(buf.shape[1:])
+    sizes = [E * K] * 3 + [E] + [E * K] * len(names)
+    parts = torch.split(buf, sizes, 0)
+    lev = [K, K, K, 1] + [K] * len(names)
+    views = [pp.view(tuple(lead) + (kk,) + tail) for pp, kk in zip(parts, lev)]
+    st = cls(views[0], views[1], views[2], views[3], dict(zip(names, views[4:])), sim_time)
+    st._flat = (buf, tuple(lead), K, tuple(names))
+    return st
+
+  def flat_buffer(self):
+    """The backing buffer if every leaf is still its view, else None."""
+    if self._flat is None:
+      return None
+    buf, lead, K, names = self._flat
+    if tuple(self.tracers.keys()) != names:
+      return None
+    E = int(np.prod(lead)) if lead else 1
+    step = buf.stride(0) * buf.element_size()
+    base = buf.data_ptr()
+    offs = [0, E * K, 2 * E * K, 3 * E * K]
+    offs += [3 * E * K + E + i * E * K for i in range(len(names))]
+    leaves = [getattr(self, f) for f in self._ARRAYS] + [self.tracers[n] for n in names]
+    for leaf, off in zip(leaves, offs):
+      if leaf.data_ptr() != base + off * step or not leaf.is_contiguous():
+        return None
+    return buf
 
   def tree_map_arrays(self, fn):
     kw = {f: fn(getattr(self, f)) for f in self._ARRAYS}
@@ -64,17 +103,23 @@ class State:
 
   def _combine(self, other, op):
     if isinstance(other, State):
+      st = None
+      if self.sim_time is not None and other.sim_time is not None:
+        st = op(self.sim_time, other.sim_time)
+      a, b = self.flat_buffer(), other.flat_buffer()
+      if a is not None and b is not None and self._flat[1:] == other._flat[1:]:
+        return State.from_flat(op(a, b), *self._flat[1:], sim_time=st)
       kw = {f: op(getattr(self, f), getattr(other, f)) for f in self._ARRAYS}
       kw['tracers'] = {k: op(v, other.tracers[k]) for k, v in self.tracers.items()}
-      if self.sim_time is None or other.sim_time is None:
-        st = None
-      else:
-        st = op(self.sim_time, other.sim_time)
       kw['sim_time'] = st
       return State(**kw)
+    st = None if self.sim_time is None else op(self.sim_time, other)
+    a = self.flat_buffer()
+    if a is not None:
+      return State.from_flat(op(a, other), *self._flat[1:], sim_time=st)
     kw = {f: op(getattr(self, f), other) for f in self._ARRAYS}
     kw['tracers'] = {k: op(v, other) for k, v in self.tracers.items()}
-    kw['sim_time'] = None if self.sim_time is None else op(self.sim_time, other)
+    kw['sim_time'] = st
     return State(**kw)
 
   def __add__(self, other):
@@ -100,7 +145,24 @@ class State:
 
   @property
   def fields(self):
-    return dataclasses.fields(self)
+    return [f for f in dataclasses.fields(self) if not f.name.startswith('_')]
+
+
+def flat_layout(state: State):
+  """(lead, K, names) of a state's leaves."""
+  lead = tuple(state.vorticity.shape[:-3])
+  return lead, state.vorticity.shape[-3], tuple(state.tracers.keys())
+
+
+def to_flat(state: State) -> State:
+  """Flat-backed version of `state` (a copy unless it already is one)."""
+  if state.flat_buffer() is not None:
+    return state
+  lead, K, names = flat_layout(state)
+  R, L = state.vorticity.shape[-2:]
+  leaves = [getattr(state, f) for f in State._ARRAYS] + [state.tracers[n] for n in names]
+  buf = torch.cat([t.reshape(-1, R, L) for t in leaves], 0)
+  return State.from_flat(buf, lead, K, names, sim_time=state.sim_time)
 
 
 def validate_state_shape(state: State, coords: coordinate_systems.CoordinateSystem):
@@ -289,6 +351,16 @@ class _ImplicitOperator:
     except Exception:  # pylint: disable=broad-except  (interpreter shutdown)
       pass
 
+  def apply_into(self, fn_name, eta, members, div, temp, lnps, od, ot, ol):
+    f = self.factor(eta, div.device)
+    code = _lib.DINO_F64 if div.dtype == torch.float64 else _lib.DINO_F32
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    stream = ctypes.c_void_p(torch.cuda.current_stream(div.device).cuda_stream)
+    with torch.cuda.device(div.device):
+      rc = getattr(_lib.lib(), fn_name)(f, p(div), p(temp), p(lnps), p(od), p(ot), p(ol),
+                                        members, code, stream)
+    _lib.check(rc, fn_name)
+
   def apply(self, fn_name, eta, div, temp, lnps):
     div = div.contiguous()
     temp = temp.contiguous()
@@ -417,6 +489,117 @@ class _NodalFused(torch.autograd.Function):
           ctypes.c_void_p(gin.data_ptr()), ctx.members, ctx.code, stream)
     _lib.check(rc, 'dino_nodal_explicit_adjoint')
     return gin, None, None
+
+
+class _Handle:
+  """Owns a library object created by dino_*_create."""
+
+  def __init__(self, handle, destroy: str):
+    self.handle = handle
+    self._destroy = destroy
+
+  def __del__(self):
+    try:
+      if _lib._lib is not None and self.handle.value:
+        getattr(_lib._lib, self._destroy)(self.handle)
+    except Exception:  # pylint: disable=broad-except  (interpreter shutdown)
+      pass
+
+
+def _launch(name, h, a, b, out, members, like):
+  code = _lib.DINO_F64 if like.dtype == torch.float64 else _lib.DINO_F32
+  stream = ctypes.c_void_p(torch.cuda.current_stream(like.device).cuda_stream)
+  ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+  with torch.cuda.device(like.device):
+    rc = getattr(_lib.lib(), name)(h.handle, ptr(a), ptr(b), ptr(out), members, code, stream)
+  _lib.check(rc, name)
+
+
+class _ModalPrep(torch.autograd.Function):
+  """State buffer -> inverse-transform input buffer (adjoint: transposed
+  stencils)."""
+
+  @staticmethod
+  def forward(ctx, buf, h, members):
+    buf = buf.contiguous()
+    n = _lib.lib().dino_modal_fields(h.handle, members, 1)
+    out = torch.empty((n,) + tuple(buf.shape[1:]), dtype=buf.dtype, device=buf.device)
+    _launch('dino_modal_prep', h, buf, None, out, members, buf)
+    ctx.h, ctx.members, ctx.shape = h, members, tuple(buf.shape)
+    return out
+
+  @staticmethod
+  def backward(ctx, g):
+    g = g.contiguous()
+    gin = torch.empty(ctx.shape, dtype=g.dtype, device=g.device)
+    _launch('dino_modal_prep_adjoint', ctx.h, g, None, gin, ctx.members, g)
+    return gin, None, None
+
+
+class _ModalEpilogue(torch.autograd.Function):
+  """Forward-transform output buffer -> state-shaped tendency buffer."""
+
+  @staticmethod
+  def forward(ctx, x, h, members, orography):
+    x = x.contiguous()
+    n = _lib.lib().dino_modal_fields(h.handle, members, 0)
+    out = torch.empty((n,) + tuple(x.shape[1:]), dtype=x.dtype, device=x.device)
+    _launch('dino_modal_epilogue', h, x, orography, out, members, x)
+    ctx.h, ctx.members, ctx.shape = h, members, tuple(x.shape)
+    return out
+
+  @staticmethod
+  def backward(ctx, g):
+    g = g.contiguous()
+    gin = torch.empty(ctx.shape, dtype=g.dtype, device=g.device)
+    _launch('dino_modal_epilogue_adjoint', ctx.h, g, None, gin, ctx.members, g)
+    return gin, None, None, None
+
+
+def _state_views(buf, members, K, nt):
+  """(div, temp, lnps) views of a flat state buffer."""
+  E = members
+  return buf[E * K:2 * E * K], buf[2 * E * K:3 * E * K], buf[3 * E * K:3 * E * K + E]
+
+
+class _ImplicitFlat(torch.autograd.Function):
+  """Implicit terms (solve=False) or (I - eta G)^{-1} (solve=True) on a flat
+  state buffer; FP64 GPU kernels."""
+
+  @staticmethod
+  def forward(ctx, buf, op, eta, members, nt, solve):
+    K = op.K
+    buf = buf.contiguous()
+    out = torch.zeros_like(buf) if not solve else torch.empty_like(buf)
+    d, t, l = _state_views(buf, members, K, nt)
+    od, ot, ol = _state_views(out, members, K, nt)
+    if solve:
+      E = members
+      out[:E * K].copy_(buf[:E * K])                  # vorticity passes through
+      out[3 * E * K + E:].copy_(buf[3 * E * K + E:])  # tracers pass through
+    op.apply_into('dino_implicit_solve' if solve else 'dino_implicit_terms', eta, members,
+                  d, t, l, od, ot, ol)
+    ctx.op, ctx.eta, ctx.members, ctx.nt, ctx.solve = op, eta, members, nt, solve
+    return out
+
+  @staticmethod
+  def backward(ctx, g):
+    op, E, K = ctx.op, ctx.members, ctx.op.K
+    g = g.contiguous()
+    gd, gt, gl = _state_views(g, E, K, ctx.nt)
+    R_, L_ = g.shape[1:]
+    shp = lambda t, kk: t.reshape(E, kk, R_, L_)
+    if ctx.solve:
+      gin = g.clone()  # vorticity and tracers pass through
+      a, b, c = op.apply_transpose(ctx.eta, shp(gd, K), shp(gt, K), shp(gl, 1))
+    else:
+      gin = torch.zeros_like(g)
+      a, b, c = op.terms_transpose(shp(gd, K), shp(gt, K), shp(gl, 1))
+    vd, vt, vl = _state_views(gin, E, K, ctx.nt)
+    vd.copy_(a.reshape(vd.shape))
+    vt.copy_(b.reshape(vt.shape))
+    vl.copy_(c.reshape(vl.shape))
+    return gin, None, None, None, None, None
 
 
 # ------------------------------------------------------------------ equations
@@ -555,6 +738,34 @@ class PrimitiveEquationsSigma:
     cache[key] = h
     return h
 
+  def _modal_handle(self, device, names):
+    key = (torch.device(device).index, tuple(names))
+    cache = self.__dict__.setdefault('_modal_handles', {})
+    h = cache.get(key)
+    if h is not None:
+      return h
+    grid = self.coords.horizontal
+    R_, L_ = grid.modal_shape
+    f64 = lambda a: np.ascontiguousarray(np.asarray(a, np.float64))
+    il = f64(spherical_harmonic._grid_host_const(grid, 'inverse_laplacian'))
+    ev = f64(grid.laplacian_eigenvalues)
+    mask = np.ones(L_)
+    mask[-1] = 0.0
+    cst = [f64(spherical_harmonic._grid_host_const(grid, n))
+           for n in ('cdlat_up', 'cdlat_dn', 'scdlat_up', 'scdlat_dn')]
+    hum = names.index(self.humidity_key) if self.humidity_key is not None else -1
+    handle = ctypes.c_void_p()
+    dp = _lib.dptr
+    with torch.cuda.device(device):
+      rc = _lib.lib().dino_modal_create(
+          R_, L_, self.coords.vertical.layers, len(names), hum, float(grid.radius),
+          float(self.physics_specs.g), dp(il), dp(ev), dp(f64(mask)), *(dp(c) for c in cst),
+          ctypes.byref(handle))
+    _lib.check(rc, 'dino_modal_create')
+    h = _Handle(handle, 'dino_modal_destroy')
+    cache[key] = h
+    return h
+
   def _check_tracers(self, state):
     names = list(state.tracers.keys())
     if self.humidity_key is not None and self.humidity_key not in state.tracers:
@@ -678,29 +889,37 @@ class PrimitiveEquationsSigma:
     """Explicit tendencies (primitive_equations.py:1126-1194)."""
     validate_state_shape(state, self.coords)
     names = self._check_tracers(state)
+    if self.fused_nodal:
+      return self._explicit_fused(state, names)
+    return self._explicit_unfused(state, names)
+
+  def _explicit_fused(self, state: State, names) -> State:
+    """prep kernel -> one inverse transform -> fused nodal kernel -> one
+    forward transform -> epilogue kernel, on flat (fields, R, L) buffers."""
     grid = self.coords.horizontal
     sh = grid.spherical_harmonics
+    flat = to_flat(state)
+    buf = flat.flat_buffer()
+    lead, K, _ = flat._flat[1:]
+    members = int(np.prod(lead)) if lead else 1
+    mh = self._modal_handle(buf.device, names)
+    prep = _ModalPrep.apply(buf, mh, members)
+    nodal = sh.inverse_transform(prep)
+    nout = _NodalFused.apply(nodal, self._nodal_handle(buf.device, names), members)
+    mod = sh.transform(nout)
+    tend = _ModalEpilogue.apply(mod, mh, members, self._orography_modal(buf).contiguous())
+    return State.from_flat(tend, lead, K, names,
+                           sim_time=None if state.sim_time is None else 1.0)
+
+  def _explicit_unfused(self, state: State, names) -> State:
+    grid = self.coords.horizontal
     ps = self.physics_specs
     a = grid.radius
     x = state.vorticity
     moist = self.humidity_key is not None
-    K = self.coords.vertical.layers
-    R_, L_ = grid.modal_shape
-    lead = tuple(x.shape[:-3])
-    members = int(np.prod(lead)) if lead else 1
     inv = self._inverse_inputs(state, names)
-    if self.fused_nodal:
-      cat = torch.cat([t.reshape(-1, R_, L_) for t in inv], 0)
-      nodal = sh.inverse_transform(cat)
-      h = self._nodal_handle(x.device, names)
-      nout = _NodalFused.apply(nodal, h, members)
-      mcat = sh.transform(nout)
-      levels = [K] * 6 + [1] + [K] * (3 * len(names)) + ([K] * 3 if moist else [])
-      parts = torch.split(mcat, [members * kk for kk in levels], 0)
-      mod = [pp.reshape(lead + (kk, R_, L_)) for pp, kk in zip(parts, levels)]
-    else:
-      nod = grid.to_nodal_batched(inv)
-      mod = grid.to_modal_batched(self._nodal_unfused(nod, names, x))
+    nod = grid.to_nodal_batched(inv)
+    mod = grid.to_modal_batched(self._nodal_unfused(nod, names, x))
     cu, cv, ke_m, temp_m, ut_m, vt_m, lsp_m = mod[:7]
 
     def div_cos(mm, nn):
@@ -737,6 +956,13 @@ class PrimitiveEquationsSigma:
     """Implicit tendencies (primitive_equations.py:1196-1249), FP64 kernel."""
     validate_state_shape(state, self.coords)
     op = self._implicit
+    buf = state.flat_buffer()
+    if buf is not None:
+      lead, K, names = state._flat[1:]
+      members = int(np.prod(lead)) if lead else 1
+      out = _ImplicitFlat.apply(buf, op, 0.0, members, len(names), False)
+      return State.from_flat(out, lead, K, names,
+                             sim_time=None if state.sim_time is None else 0.0)
     d, t, l = _ImplicitTerms.apply(state.divergence, state.temperature_variation,
                                    state.log_surface_pressure, op, 0.0)
     return State(
@@ -756,6 +982,12 @@ class PrimitiveEquationsSigma:
           'Pass a Python float: the inverse is factorised in FP64 for each static step.')
     validate_state_shape(state, self.coords)
     op = self._implicit
+    buf = state.flat_buffer()
+    if buf is not None:
+      lead, K, names = state._flat[1:]
+      members = int(np.prod(lead)) if lead else 1
+      out = _ImplicitFlat.apply(buf, op, float(step_size), members, len(names), True)
+      return State.from_flat(out, lead, K, names, sim_time=state.sim_time)
     d, t, l = _ImplicitSolve.apply(state.divergence, state.temperature_variation,
                                    state.log_surface_pressure, op, op, float(step_size))
     return State(state.vorticity, d, t, l, dict(state.tracers), sim_time=state.sim_time)

@@ -182,6 +182,8 @@ def sht_call(fn_name: str, plan: Plan, x: torch.Tensor, out_tail: tuple, precisi
   if batch == 0:
     return out
   work = _workspace(plan, batch, x)
+  if x.dtype == torch.float64:
+    precision = _lib.PREC_NATIVE  # FP64 data always runs FP64 arithmetic
   fn = getattr(_lib.lib(), fn_name)
   with torch.cuda.device(x.device):
     rc = fn(plan.handle, _ptr(x), _ptr(out), _ptr(work), batch, _dtype_code(x), precision,

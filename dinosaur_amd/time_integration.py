@@ -221,6 +221,9 @@ def trajectory_from_step(step_fn, outer_steps: int, inner_steps: int, *,
 
 def _flatten_state(x):
   """State -> (list of tensors, rebuild)."""
+  if hasattr(x, '_ARRAYS') and x.flat_buffer() is not None:
+    layout, st, cls = x._flat[1:], x.sim_time, type(x)
+    return [x.flat_buffer()], lambda ls: cls.from_flat(ls[0], *layout, sim_time=st)
   if hasattr(x, '_ARRAYS'):
     names = list(x._ARRAYS)
     keys = list(x.tracers.keys())

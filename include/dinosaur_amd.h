@@ -145,8 +145,8 @@ int dino_implicit_create(int K, int L, int R, const double* G, const double* H,
                          double eta, dino_implicit** out);
 int dino_implicit_destroy(dino_implicit* f);
 
-/* In place? No: reads div/temp (members, K, R, L) and lnps (members, 1, R, L)
- * and writes the three outputs (which may not alias the inputs). */
+/* Reads div/temp (members, K, R, L) and lnps (members, 1, R, L) and writes
+ * the three outputs, which must not alias the inputs. */
 int dino_implicit_solve(const dino_implicit* f, const void* div,
                         const void* temp, const void* lnps, void* div_out,
                         void* temp_out, void* lnps_out, int64_t members,
@@ -199,6 +199,40 @@ int dino_nodal_explicit(const dino_nodal* n, const void* nodal_in,
  * reference obtains this by XLA autodiff of the same elementwise graph). */
 int dino_nodal_explicit_adjoint(const dino_nodal* n, const void* nodal_in,
                                 const void* out_cot, void* in_cot,
+                                int64_t members, int dtype, void* stream);
+
+/* Fused modal-space stages of PrimitiveEquationsSigma.explicit_terms.
+ * State buffers are (fields, 2M-1, L) with groups vorticity (K),
+ * divergence (K), temperature_variation (K), log_surface_pressure (1),
+ * tracers (K each), each (members, levels).  dino_modal_prep writes the
+ * inverse-transform input groups of dino_nodal_explicit; dino_modal_epilogue
+ * reads the forward-transform output groups of dino_nodal_explicit and
+ * writes the state-shaped tendency (orography: (2M-1, L) modal orography or
+ * NULL).  The adjoints map cotangents back.  Replaces get_cos_lat_vector,
+ * cos_lat_grad (spherical_harmonic.py:1104-1178) and the modal combination
+ * of explicit_terms (primitive_equations.py:1001-1059, 778-805, 861-963,
+ * 1126-1194).  Coefficients: inv_lap, lap, clip_mask of length L; up, dn
+ * ((l+1) a_lm, -l b_lm) and up2, dn2 ((l-1) a_lm, -(l+2) b_lm) of shape
+ * (2M-1, L), from Grid._derivative_recurrence_weights (:1063-1070). */
+typedef struct dino_modal dino_modal;
+int dino_modal_create(int R, int L, int K, int n_tracers, int humidity_index,
+                      double radius, double gravity, const double* inv_lap,
+                      const double* lap, const double* clip_mask,
+                      const double* up, const double* dn, const double* up2,
+                      const double* dn2, dino_modal** out);
+int dino_modal_destroy(dino_modal* m);
+/* Fields of the state (which = 0), prep output (1) or epilogue input (2). */
+int64_t dino_modal_fields(const dino_modal* m, int64_t members, int which);
+int dino_modal_prep(const dino_modal* m, const void* state, const void* unused,
+                    void* out, int64_t members, int dtype, void* stream);
+int dino_modal_prep_adjoint(const dino_modal* m, const void* out_cot,
+                            const void* unused, void* state_cot,
+                            int64_t members, int dtype, void* stream);
+int dino_modal_epilogue(const dino_modal* m, const void* in,
+                        const void* orography, void* tendency, int64_t members,
+                        int dtype, void* stream);
+int dino_modal_epilogue_adjoint(const dino_modal* m, const void* tendency_cot,
+                                const void* unused, void* in_cot,
                                 int64_t members, int dtype, void* stream);
 
 #ifdef __cplusplus

@@ -52,7 +52,36 @@ def _parse():
   ap.add_argument('--no-plain', action='store_true')
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--forward-only', action='store_true')
+  ap.add_argument('--kernel-only', default=None, choices=STAGES,
+                  help='run one stage kernel at the bench shape, then a 256 MiB device '
+                       'copy (the calibration kernel for hardware byte counters), and exit')
+  ap.add_argument('--reps', type=int, default=10)
   return ap.parse_args()
+
+
+STAGES = ('legendre_forward', 'legendre_inverse', 'dft_forward', 'dft_inverse')
+# fields per member in one explicit-terms evaluation of the moist L37 state
+FWD_FIELDS, INV_FIELDS = 667, 374
+
+
+def kernel_only(torch, stage, precision, members, reps):
+  from dinosaur_amd import _lib
+  from dinosaur_amd import spherical_harmonic as sh
+  grid = getattr(sh.Grid, GRID)()
+  batch = members * (FWD_FIELDS if stage.endswith('forward') else INV_FIELDS)
+  secs, work, kind = measure_kernel(torch, grid, batch, stage, _lib.PRECISIONS[precision],
+                                    reps)
+  unit = 'TFLOP/s' if kind == 'flop' else 'GB/s'
+  rate = work / secs / (1e12 if kind == 'flop' else 1e9)
+  print(f'[bench] {stage} batch={batch}: {secs * 1e3:.3f} ms/launch, {rate:.1f} {unit} '
+        f'algorithmic', file=sys.stderr, flush=True)
+  src = torch.ones(64 << 20, device='cuda')
+  dst = torch.empty_like(src)
+  for _ in range(3):
+    dst.copy_(src)
+  torch.cuda.synchronize()
+  print('[bench] calibration copy: 268435456 bytes read, 268435456 written per launch',
+        file=sys.stderr, flush=True)
 
 
 def _sync(torch):
@@ -163,11 +192,10 @@ class DenseFP32Harmonics:
     return _Dense(**kw)
 
 
-def measure_kernel(torch, coords, batch, stage, precision_code, reps=10):
+def measure_kernel(torch, grid, batch, stage, precision_code, reps=10):
   """Average duration of one stage kernel launch (HIP events on the launch
   stream) and its algorithmic work."""
   from dinosaur_amd import _lib
-  grid = coords.horizontal
   sh = grid.spherical_harmonics
   dev = torch.device('cuda', torch.cuda.current_device())
   plan = sh.plan(dev)
@@ -226,7 +254,7 @@ def cpu_baseline(sample_fields=37):
   t2 = time.perf_counter()
   per_inv = (t1 - t0) / sample_fields
   per_fwd = (t2 - t1) / sample_fields
-  n_inv, n_fwd = 374, 667  # field transforms per explicit evaluation (L37, 3 tracers)
+  n_inv, n_fwd = INV_FIELDS, FWD_FIELDS  # field transforms per explicit evaluation
   per_member_step = 3 * (n_inv * per_inv + n_fwd * per_fwd)
   cores = torch.get_num_threads()
   try:
@@ -259,6 +287,9 @@ def main():
   precision = args.precision
   if precision == 'auto':
     precision = 'bf16x3' if _lib.lib() and _split_available(torch) else 'float32'
+  if args.kernel_only:
+    kernel_only(torch, args.kernel_only, precision, args.members, args.reps)
+    return
   members = args.members
   print(f'[bench] chosen leg: {GRID} L{LAYERS} moist+3 tracers, {members} members, '
         f'precision={precision}', file=sys.stderr, flush=True)
@@ -273,11 +304,11 @@ def main():
 
   # dominant kernel, timed live on its launch stream
   pcode = _lib.PRECISIONS[precision]
-  fwd_batch = members * 667
+  fwd_batch = members * FWD_FIELDS
   kern = {}
-  for stage in ('legendre_forward', 'legendre_inverse', 'dft_forward', 'dft_inverse'):
-    secs, work, kind = measure_kernel(torch, coords, fwd_batch if stage.endswith('forward')
-                                      else members * 374, stage, pcode)
+  for stage in STAGES:
+    secs, work, kind = measure_kernel(torch, coords.horizontal, fwd_batch if stage.endswith('forward')
+                                      else members * INV_FIELDS, stage, pcode)
     kern[stage] = (secs, work, kind)
     print(f'[bench] {stage}: {secs * 1e3:.3f} ms/launch', file=sys.stderr, flush=True)
   dom = max(kern, key=lambda k: kern[k][0])

@@ -61,6 +61,13 @@ struct dino_plan {
   uint16_t* pbf_lo;
   double* tw64;         // (I) complex twiddles exp(-2 pi i k / I) as (cos, sin)
   float* tw32;
+  // Forward-transform tables in MFMA fragment order (legendre_rows.hip).
+  // fks = number of 32-node reduction steps the tables hold (1, 2, 4 or 8;
+  // 0 when Jh > 256 and the tiled kernels are used instead).
+  int fks;
+  int64_t* ftile_off;   // (M) first tile of each m, in 16-byte units
+  float* ffrag32;       // FP32 fragments
+  uint16_t* ffragbf;    // BF16 hi/lo fragments
 };
 
 struct dino_implicit {

@@ -356,6 +356,9 @@ int inv_by_width(const dino_plan* pl, const float* in, float* out, int64_t B, hi
 
 }  // namespace
 
+int legendre_forward_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
+                          bool split, hipStream_t s);
+
 int legendre_inverse_mfma(const dino_plan* pl, const void* modal, void* fourier, int64_t B,
                           int precision, hipStream_t s) {
   if (B == 0) return DINO_OK;
@@ -374,9 +377,11 @@ int legendre_inverse_mfma(const dino_plan* pl, const void* modal, void* fourier,
 int legendre_forward_mfma(const dino_plan* pl, const void* fourier, void* modal, int64_t B,
                           int precision, hipStream_t s) {
   if (B == 0) return DINO_OK;
-  DINO_REQUIRE((2 * B + TR - 1) / TR <= 65535, DINO_EINVAL, "batch too large");
   const float* in = static_cast<const float*>(fourier);
   float* out = static_cast<float*>(modal);
+  if (pl->fks > 0 && (precision == DINO_PREC_NATIVE || precision == DINO_PREC_BF16X3))
+    return legendre_forward_rows(pl, in, out, B, precision == DINO_PREC_BF16X3, s);
+  DINO_REQUIRE((2 * B + TR - 1) / TR <= 65535, DINO_EINVAL, "batch too large");
   switch (precision) {
     case DINO_PREC_NATIVE: return launch_fwd<PolicyF32, 32>(pl, in, out, B, s);
     case DINO_PREC_BF16X3: return launch_fwd<PolicySplit<true>, 32>(pl, in, out, B, s);

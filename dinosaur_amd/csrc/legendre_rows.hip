@@ -1,0 +1,361 @@
+// Forward Legendre contraction with register-resident rows.
+//
+// D[row, l] = sum_jj X_{(l-m) mod 2}[row, jj] P[l, jj], X_0 = f_up + f_dn,
+// X_1 = f_up - f_dn (equatorial fold; same contraction as legendre.hip).
+//
+// The contraction is HBM-bound at its batch sizes (one read of the Fourier
+// rows, one write of the modal rows, including the zero block l < m), so
+// the kernel is laid out to touch each byte of HBM once:
+//   * a workgroup owns 64 rows (4 waves x 16) of one zonal wavenumber m and
+//     sweeps every degree tile of that m, so the Fourier rows are read once;
+//   * each wave keeps the folded rows of its 16-row strip in VGPRs as MFMA
+//     A-operand fragments for the whole reduction (all Jh northern nodes),
+//     split once into BF16 hi/lo for the split-precision path;
+//   * the Legendre values arrive through a double-buffered LDS tile that is
+//     filled from a plan table stored in fragment order (16-byte lane reads,
+//     conflict-free) -- build_forward_fragments below.
+//
+// Reduction order inside a 16-node group t: lane group g = lane / 16 holds
+// nodes jj = 16 t + 4 g + e (e = 0..3), one float4 of the row per lane.  A
+// degree tile covers 32 consecutive degrees lt .. lt + 31 with lt a multiple
+// of 32; accumulator q (0 or 1) holds the degrees lt + 2 c + q, c = lane % 16.
+#include "dino_common.h"
+
+namespace dino {
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kRowsPerBlock = 64;
+constexpr int kTileDeg = 32;
+
+__device__ __forceinline__ void row_field(int m, int q, int* b, int* r) {
+  if (m == 0) {
+    *b = q;
+    *r = 0;
+  } else {
+    *b = q >> 1;
+    *r = 2 * m - 1 + (q & 1);
+  }
+}
+
+__host__ __device__ inline int tiles_of_m(int m, int L) {
+  return (L - (m / kTileDeg) * kTileDeg + kTileDeg - 1) / kTileDeg;
+}
+
+// float4 units per degree tile: 2 parities x 2 KS node groups x 64 lanes for
+// FP32, 2 parities x KS steps x (hi, lo) x 64 lanes for BF16 -- both 256 KS.
+__host__ __device__ constexpr int tile_f4(int ks) { return 256 * ks; }
+
+__device__ __forceinline__ bf16x8 split_hi(const float4& a, const float4& b) {
+  bf16x8 h;
+  h[0] = (__bf16)a.x; h[1] = (__bf16)a.y; h[2] = (__bf16)a.z; h[3] = (__bf16)a.w;
+  h[4] = (__bf16)b.x; h[5] = (__bf16)b.y; h[6] = (__bf16)b.z; h[7] = (__bf16)b.w;
+  return h;
+}
+
+__device__ __forceinline__ bf16x8 split_lo(const float4& a, const float4& b, const bf16x8& h) {
+  bf16x8 l;
+  l[0] = (__bf16)(a.x - (float)h[0]); l[1] = (__bf16)(a.y - (float)h[1]);
+  l[2] = (__bf16)(a.z - (float)h[2]); l[3] = (__bf16)(a.w - (float)h[3]);
+  l[4] = (__bf16)(b.x - (float)h[4]); l[5] = (__bf16)(b.y - (float)h[5]);
+  l[6] = (__bf16)(b.z - (float)h[6]); l[7] = (__bf16)(b.w - (float)h[7]);
+  return l;
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(const floatx4& v) {
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ __forceinline__ floatx4 ld4(const float4* p) {
+  return *reinterpret_cast<const floatx4*>(p);
+}
+
+// VEC: J % 8 == 0, L % 4 == 0 and 16-byte aligned buffers (float4 row loads,
+// float2 / float4 stores); otherwise scalar accesses, any J and L.
+template <bool BF, int KS, bool VEC>
+__global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
+    const float* __restrict__ fourier, float* __restrict__ modal,
+    const float4* __restrict__ ftab, const int64_t* __restrict__ ftile_off, int B, int L,
+    int J, int Jh, int Rrows) {
+  constexpr int KT = 2 * KS;  // 16-node groups
+  constexpr int TF4 = tile_f4(KS);
+  constexpr int PER = TF4 / 256;
+  __shared__ floatx4 pbuf[2][TF4];
+  const int m = blockIdx.y;
+  const int nrows = (m == 0) ? B : 2 * B;
+  const int row0 = blockIdx.x * kRowsPerBlock;
+  if (row0 >= nrows) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, c16 = lane & 15;
+  const int lt0 = (m / kTileDeg) * kTileDeg;
+  const int ntile = tiles_of_m(m, L);
+  const float4* tb = ftab + ftile_off[m];
+
+  // Table tile prefetch (clang vector type, unconditional loads: both keep
+  // the staging array in VGPRs so the loads stay asynchronous).
+  floatx4 st[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) st[i] = ld4(tb + tid + 256 * i);
+
+  // Folded rows: X0 pairs with the even-d degrees of a tile (relative
+  // parity (m & 1)), X1 with the odd-d ones.  For odd m the roles of the
+  // sum and the difference swap.
+  // Loads are unconditional from clamped addresses (invalid slots are zeroed
+  // afterwards by selects), so all 2 KT loads of a lane issue back to back.
+  float4 x0[KT], x1[KT];
+  {
+    const int row = row0 + wave * 16 + c16;
+    const bool rok = row < nrows;
+    const float sgn = (m & 1) ? -1.f : 1.f;
+    int b = 0, r = 0;
+    if (rok) row_field(m, row, &b, &r);
+    const float* f = fourier + ((int64_t)b * Rrows + r) * J;
+    const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+      const int jj = 16 * t + 4 * g;
+      floatx4 u, d;
+      if constexpr (VEC) {
+        const bool ok = rok && jj < Jh;
+        const floatx4 uu = *reinterpret_cast<const floatx4*>(f + (ok ? J - Jh + jj : 0));
+        const floatx4 dr = *reinterpret_cast<const floatx4*>(f + (ok ? Jh - 4 - jj : 0));
+        u = ok ? uu : zero;
+        d = ok ? floatx4{dr[3], dr[2], dr[1], dr[0]} : zero;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int q = jj + e;
+          const bool ok = rok && q < Jh;
+          // odd J: node jj = 0 is the equator, counted once
+          const bool okd = ok && !((J & 1) && q == 0);
+          const float uu = f[ok ? J - Jh + q : 0];
+          const float dd = f[okd ? Jh - 1 - q : 0];
+          u[e] = ok ? uu : 0.f;
+          d[e] = okd ? dd : 0.f;
+        }
+      }
+      x0[t] = make_float4(u[0] + sgn * d[0], u[1] + sgn * d[1], u[2] + sgn * d[2],
+                          u[3] + sgn * d[3]);
+      x1[t] = make_float4(u[0] - sgn * d[0], u[1] - sgn * d[1], u[2] - sgn * d[2],
+                          u[3] - sgn * d[3]);
+    }
+  }
+  bf16x8 h0[BF ? KS : 1], l0[BF ? KS : 1], h1[BF ? KS : 1], l1[BF ? KS : 1];
+  if constexpr (BF) {
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      h0[s] = split_hi(x0[2 * s], x0[2 * s + 1]);
+      l0[s] = split_lo(x0[2 * s], x0[2 * s + 1], h0[s]);
+      h1[s] = split_hi(x1[2 * s], x1[2 * s + 1]);
+      l1[s] = split_lo(x1[2 * s], x1[2 * s + 1], h1[s]);
+    }
+  }
+
+  // Zero block: whole 32-degree blocks below m.
+  if (lt0 > 0) {
+    if constexpr (VEC) {
+      const int n4 = lt0 / 4;
+      for (int idx = tid; idx < kRowsPerBlock * n4; idx += 256) {
+        const int q = idx / n4, c = idx - q * n4;
+        const int row = row0 + q;
+        if (row >= nrows) continue;
+        int b, r;
+        row_field(m, row, &b, &r);
+        *reinterpret_cast<float4*>(modal + ((int64_t)b * Rrows + r) * L + 4 * c) =
+            make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    } else {
+      for (int idx = tid; idx < kRowsPerBlock * lt0; idx += 256) {
+        const int q = idx / lt0, c = idx - q * lt0;
+        const int row = row0 + q;
+        if (row >= nrows) continue;
+        int b, r;
+        row_field(m, row, &b, &r);
+        modal[((int64_t)b * Rrows + r) * L + c] = 0.f;
+      }
+    }
+  }
+
+  // Output rows of this lane: 4 g + i of the wave's strip.
+  float* orow[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int row = row0 + wave * 16 + 4 * g + i;
+    orow[i] = nullptr;
+    if (row < nrows) {
+      int b, r;
+      row_field(m, row, &b, &r);
+      orow[i] = modal + ((int64_t)b * Rrows + r) * L;
+    }
+  }
+
+  for (int tt = 0; tt < ntile; ++tt) {
+    floatx4* buf = pbuf[tt & 1];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) buf[tid + 256 * i] = st[i];
+    __syncthreads();
+    {
+      // the last iteration reloads its own tile, which is never stored
+      const float4* nb = tb + (int64_t)(tt + 1 < ntile ? tt + 1 : tt) * TF4;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) st[i] = ld4(nb + tid + 256 * i);
+    }
+    floatx4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0;
+    if constexpr (BF) {
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const bf16x8 bh0 = as_bf16x8(buf[((0 * KS + s) * 2 + 0) * 64 + lane]);
+        const bf16x8 bl0 = as_bf16x8(buf[((0 * KS + s) * 2 + 1) * 64 + lane]);
+        const bf16x8 bh1 = as_bf16x8(buf[((1 * KS + s) * 2 + 0) * 64 + lane]);
+        const bf16x8 bl1 = as_bf16x8(buf[((1 * KS + s) * 2 + 1) * 64 + lane]);
+        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l0[s], bh0, a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l1[s], bh1, a1, 0, 0, 0);
+        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h0[s], bl0, a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h1[s], bl1, a1, 0, 0, 0);
+        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h0[s], bh0, a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h1[s], bh1, a1, 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < KT; ++t) {
+        const floatx4 p0 = buf[(0 * KT + t) * 64 + lane];
+        const floatx4 p1 = buf[(1 * KT + t) * 64 + lane];
+        a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0[t].x, p0[0], a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1[t].x, p1[0], a1, 0, 0, 0);
+        a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0[t].y, p0[1], a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1[t].y, p1[1], a1, 0, 0, 0);
+        a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0[t].z, p0[2], a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1[t].z, p1[2], a1, 0, 0, 0);
+        a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(x0[t].w, p0[3], a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1[t].w, p1[3], a1, 0, 0, 0);
+      }
+    }
+    const int l = lt0 + tt * kTileDeg + 2 * c16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (!orow[i]) continue;
+      if constexpr (VEC) {
+        if (l < L) *reinterpret_cast<float2*>(orow[i] + l) = make_float2(a0[i], a1[i]);
+      } else {
+        if (l < L) orow[i][l] = a0[i];
+        if (l + 1 < L) orow[i][l + 1] = a1[i];
+      }
+    }
+  }
+}
+
+// One thread per fragment slot of every tile of one m (blockIdx.y = m).
+// FP32 slot order [tile][q][t][lane][e]; BF16 [tile][q][s][h][lane][e'].
+__global__ void forward_fragments_kernel(int L, int Jh, int Jhp, int ks,
+                                         const double* __restrict__ p64,
+                                         const int64_t* __restrict__ m_off,
+                                         const int64_t* __restrict__ ftile_off,
+                                         float* __restrict__ f32, uint16_t* __restrict__ fbf) {
+  const int m = blockIdx.y;
+  const int64_t per_tile = 1024 * (int64_t)ks;  // slots (floats) per tile
+  const int64_t n = per_tile * tiles_of_m(m, L);
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int tile = (int)(idx / per_tile);
+  const int rem = (int)(idx - tile * per_tile);
+  const int lt = (m / kTileDeg) * kTileDeg + kTileDeg * tile;
+  auto value = [&](int l, int jj) -> double {
+    return (l >= m && l < L && jj < Jh) ? p64[m_off[m] + (int64_t)(l - m) * Jhp + jj] : 0.0;
+  };
+  const int64_t base = ftile_off[m] * 4 + (int64_t)tile * per_tile;  // floats
+  {
+    // FP32: rem = ((q * KT + t) * 64 + lane) * 4 + e
+    const int KT = 2 * ks;
+    const int q = rem / (KT * 256), r2 = rem % (KT * 256);
+    const int t = r2 / 256, lane = (r2 % 256) / 4, e = r2 % 4;
+    const int l = lt + 2 * (lane % 16) + q;
+    const int jj = 16 * t + 4 * (lane / 16) + e;
+    f32[base + rem] = (float)value(l, jj);
+  }
+  {
+    // BF16: rem = ((q * KS + s) * 64 + lane) * 8 + e'; hi at h = 0, lo at h = 1
+    const int q = rem / (ks * 512), r2 = rem % (ks * 512);
+    const int s = r2 / 512, lane = (r2 % 512) / 8, e = r2 % 8;
+    const int l = lt + 2 * (lane % 16) + q;
+    const int jj = 16 * (2 * s + e / 4) + 4 * (lane / 16) + e % 4;
+    const double v = value(l, jj);
+    const __bf16 hi = (__bf16)(float)v;
+    const __bf16 lo = (__bf16)(float)(v - (double)(float)hi);
+    const int64_t ub = base * 2;  // uint16 units
+    const int64_t pos = ((int64_t)(q * ks + s) * 2) * 512 + lane * 8 + e;
+    fbf[ub + pos] = __builtin_bit_cast(uint16_t, hi);
+    fbf[ub + pos + 512] = __builtin_bit_cast(uint16_t, lo);
+  }
+}
+
+template <bool BF, int KS>
+int launch_rows(const dino_plan* pl, const float* in, float* out, int64_t B, hipStream_t s) {
+  const bool aligned = (reinterpret_cast<uintptr_t>(in) % 16 == 0) &&
+                       (reinterpret_cast<uintptr_t>(out) % 16 == 0);
+  const bool vec = pl->J % 8 == 0 && pl->L % 4 == 0 && aligned;
+  const dim3 grid((unsigned)((2 * B + kRowsPerBlock - 1) / kRowsPerBlock), (unsigned)pl->M);
+  const float4* tab = BF ? reinterpret_cast<const float4*>(pl->ffragbf)
+                         : reinterpret_cast<const float4*>(pl->ffrag32);
+  if (vec)
+    leg_fwd_rows_kernel<BF, KS, true><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, (int)B,
+                                                            pl->L, pl->J, pl->Jh, pl->R);
+  else
+    leg_fwd_rows_kernel<BF, KS, false><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, (int)B,
+                                                             pl->L, pl->J, pl->Jh, pl->R);
+  DINO_LAUNCH_CHECK("legendre forward rows kernel");
+  return DINO_OK;
+}
+
+template <bool BF>
+int rows_by_ks(const dino_plan* pl, const float* in, float* out, int64_t B, hipStream_t s) {
+  switch (pl->fks) {
+    case 1: return launch_rows<BF, 1>(pl, in, out, B, s);
+    case 2: return launch_rows<BF, 2>(pl, in, out, B, s);
+    case 4: return launch_rows<BF, 4>(pl, in, out, B, s);
+    case 8: return launch_rows<BF, 8>(pl, in, out, B, s);
+    default:
+      set_error("forward fragment tables hold %d steps", pl->fks);
+      return DINO_EINVAL;
+  }
+}
+
+}  // namespace
+
+// Number of 32-node steps the fragment tables use for this plan (a power of
+// two covering Jh), or 0 when Jh is too large for register-resident rows.
+int forward_fragment_steps(int Jh) {
+  int ks = 1;
+  while (32 * ks < Jh) ks *= 2;
+  return ks <= 8 ? ks : 0;
+}
+
+// Float4 units of the fragment tables; fills off[m] (float4 units).
+int64_t forward_fragment_layout(int M, int L, int ks, int64_t* off) {
+  int64_t acc = 0;
+  for (int m = 0; m < M; ++m) {
+    off[m] = acc;
+    acc += (int64_t)tiles_of_m(m, L) * tile_f4(ks);
+  }
+  return acc;
+}
+
+void forward_fragment_fill(const dino_plan* pl, hipStream_t s) {
+  const int64_t most = 1024 * (int64_t)pl->fks * tiles_of_m(0, pl->L);
+  const dim3 grid((unsigned)((most + 255) / 256), (unsigned)pl->M);
+  forward_fragments_kernel<<<grid, 256, 0, s>>>(pl->L, pl->Jh, pl->Jhp, pl->fks, pl->p64,
+                                                pl->m_offset, pl->ftile_off, pl->ffrag32,
+                                                pl->ffragbf);
+}
+
+bool forward_rows_available(const dino_plan* pl) { return pl->fks > 0; }
+
+int legendre_forward_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
+                          bool split, hipStream_t s) {
+  if (B == 0) return DINO_OK;
+  DINO_REQUIRE(2 * B / kRowsPerBlock < (int64_t)1 << 30, DINO_EINVAL, "batch too large");
+  return split ? rows_by_ks<true>(pl, in, out, B, s) : rows_by_ks<false>(pl, in, out, B, s);
+}
+
+}  // namespace dino

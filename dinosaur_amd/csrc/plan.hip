@@ -172,11 +172,15 @@ __global__ void quad_weights_kernel(int I, int J, const double* w, double* q64,
   q32[j] = (float)q;
 }
 
+int forward_fragment_steps(int Jh);
+int64_t forward_fragment_layout(int M, int L, int ks, int64_t* off);
+void forward_fragment_fill(const dino_plan* pl, hipStream_t s);
+
 static void free_plan(dino_plan* p) {
   if (!p) return;
   void* bufs[] = {p->nodes64, p->weights64, p->qw32, p->qw64, p->m_offset,
                   p->p64, p->p32, p->p16_hi, p->p16_lo, p->pbf_hi, p->pbf_lo,
-                  p->tw64, p->tw32};
+                  p->tw64, p->tw32, p->ftile_off, p->ffrag32, p->ffragbf};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete p;
@@ -282,6 +286,17 @@ extern "C" int dino_plan_create(int M, int L, int I, int J, const double* nodes,
   PLAN_HIP(hipGetLastError());
   twiddle_kernel<<<(I + 255) / 256, 256>>>(I, p->tw64, p->tw32);
   PLAN_HIP(hipGetLastError());
+  p->fks = forward_fragment_steps(p->Jh);
+  if (p->fks > 0) {
+    std::vector<int64_t> foff(M);
+    const int64_t f4 = forward_fragment_layout(M, L, p->fks, foff.data());
+    PLAN_HIP(hipMalloc(&p->ftile_off, sizeof(int64_t) * M));
+    PLAN_HIP(hipMalloc(&p->ffrag32, 16 * f4));
+    PLAN_HIP(hipMalloc(&p->ffragbf, 16 * f4));
+    PLAN_HIP(hipMemcpy(p->ftile_off, foff.data(), sizeof(int64_t) * M, hipMemcpyHostToDevice));
+    forward_fragment_fill(p, 0);
+    PLAN_HIP(hipGetLastError());
+  }
   PLAN_HIP(hipDeviceSynchronize());
 #undef PLAN_HIP
   *out = p;

@@ -52,7 +52,7 @@ def _parse():
   ap.add_argument('--no-plain', action='store_true')
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--forward-only', action='store_true')
-  ap.add_argument('--kernel-only', default=None, choices=STAGES,
+  ap.add_argument('--kernel-only', default=None, choices=STAGES + IMPLICIT_STAGES,
                   help='run one stage kernel at the bench shape, then a 256 MiB device '
                        'copy (the calibration kernel for hardware byte counters), and exit')
   ap.add_argument('--reps', type=int, default=10)
@@ -60,13 +60,46 @@ def _parse():
 
 
 STAGES = ('legendre_forward', 'legendre_inverse', 'dft_forward', 'dft_inverse')
+IMPLICIT_STAGES = ('implicit_solve', 'implicit_terms')
 # fields per member in one explicit-terms evaluation of the moist L37 state
 FWD_FIELDS, INV_FIELDS = 667, 374
+
+
+def measure_implicit(torch, eq, state, stage, reps=10):
+  """Average duration of one implicit kernel launch over the ensemble state
+  (HIP events on the launch stream) and its algorithmic HBM bytes."""
+  from dinosaur_amd import primitive_equations as pe
+  flat = pe.to_flat(state)
+  d, t, p = flat.divergence, flat.temperature_variation, flat.log_surface_pressure
+  outs = [torch.empty_like(d), torch.empty_like(t), torch.empty_like(p)]
+  members = int(np.prod(d.shape[:-3])) if d.ndim > 3 else 1
+  fn = 'dino_implicit_solve' if stage == 'implicit_solve' else 'dino_implicit_terms'
+  op = eq._implicit
+  call = lambda: op.apply_into(fn, 0.05, members, d, t, p, *outs)
+  call()
+  torch.cuda.synchronize()
+  stream = torch.cuda.current_stream()
+  start = torch.cuda.Event(enable_timing=True)
+  end = torch.cuda.Event(enable_timing=True)
+  start.record(stream)
+  for _ in range(reps):
+    call()
+  end.record(stream)
+  end.synchronize()
+  secs = start.elapsed_time(end) / 1e3 / reps
+  nbytes = 2.0 * (d.numel() + t.numel() + p.numel()) * d.element_size()
+  return secs, nbytes, 'byte'
 
 
 def kernel_only(torch, stage, precision, members, reps):
   from dinosaur_amd import _lib
   from dinosaur_amd import spherical_harmonic as sh
+  if stage in IMPLICIT_STAGES:
+    eq, state, _, _, _ = build(torch, precision, members)
+    secs, work, _ = measure_implicit(torch, eq, state, stage, reps)
+    print(f'[bench] {stage} members={members}: {secs * 1e3:.3f} ms/launch, '
+          f'{work / secs / 1e9:.1f} GB/s algorithmic', file=sys.stderr, flush=True)
+    return
   grid = getattr(sh.Grid, GRID)()
   batch = members * (FWD_FIELDS if stage.endswith('forward') else INV_FIELDS)
   secs, work, kind = measure_kernel(torch, grid, batch, stage, _lib.PRECISIONS[precision],

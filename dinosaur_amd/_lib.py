@@ -51,6 +51,14 @@ SIGNATURES = {
     'dino_implicit_destroy': (_c_i, [_c_p]),
     'dino_implicit_solve': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
     'dino_implicit_terms': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
+    'dino_imex_stage': (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_i, _c_p, _c_p, _c_d, _c_p, _c_i64, _c_i,
+                               _c_p]),
+    'dino_imex_stage_adjoint': (_c_i, [_c_p, _c_i, _c_p, _c_i, _c_p, _c_d, _c_p, _c_p, _c_i64,
+                                       _c_i, _c_p]),
+    'dino_implicit_solve_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i,
+                                           _c_p]),
+    'dino_implicit_terms_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i,
+                                           _c_p]),
     'dino_nodal_create': (_c_i, [_c_i, _c_i, _c_i, _c_i, _c_i, ctypes.POINTER(_c_i), _c_i, _c_i,
                                  _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp,
                                  _c_d, _c_d, _c_d, _c_d, _c_d, ctypes.POINTER(_c_p)]),

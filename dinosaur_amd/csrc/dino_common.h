@@ -82,4 +82,7 @@ struct dino_implicit {
   double* dsigma;
   double* d;
   double* lam;
+  // fragment tables and vectors of the MFMA kernels, forward [0..7] and
+  // transposed [8..14] (layout in implicit_mfma.hip)
+  double* mtab[15];
 };

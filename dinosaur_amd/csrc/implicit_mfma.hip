@@ -1,0 +1,363 @@
+// Semi-implicit solve and implicit terms on the FP64 matrix cores.
+//
+// Per column (member, modal row, degree l) of the flat state the solve is
+//   z       = diag(1 / (1 - eta^2 lam_l d)) (W delta - eta lam_l (W G T + w lnps))
+//   delta'' = V z,   T'' = T - eta (H V) z,   lnps'' = lnps - eta (dsigma^T V) z
+// with A = G H + R T_ref dsigma^T = V diag(d) W (W = V^{-1}), w = W R T_ref
+// (the same elimination as implicit.hip, primitive_equations.py:544-598,
+// 1273-1330).  The implicit terms are
+//   div = -lam_l (G T + R T_ref lnps),  temp = -H delta,  lnps = -dsigma . delta.
+//
+// All matrices are l-independent, so a batch of 16 columns is a pair of small
+// FP64 GEMMs (v_mfma_f64_16x16x4f64).  A wave owns 16 columns:
+//   stage 1: C_a = W [delta], C_b = (W G) [T] -- B operand lane (column j =
+//            lane % 16, k-group kg = lane / 16) holds rows 4 s + kg;
+//   stage 2: [V; H V; dsigma^T V] z -- the stage-1 accumulators are the B
+//            operand as they stand: f64 accumulator register i of row tile it
+//            holds row 16 it + 4 i + kg, which is row 4 s + kg of k-step
+//            s = 4 it + i.
+// Matrix operands come from fragment-ordered tables built at create time (one
+// 8-byte load per lane per MFMA, 512 contiguous bytes per wave).  Levels are
+// padded to KP = 4 ceil(K / 4); the H V block starts at row KP, so the
+// temperature fragments a lane loaded for stage 1 are the ones its stage-2
+// output rows need.
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "dino_common.h"
+
+namespace dino {
+namespace {
+
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ doublex4 mma64(double a, double b, doublex4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// NS = KP / 4 reduction steps per stage-1 block; T1 = row tiles of stage 1,
+// T2 = row tiles of stage 2 (2 KP + 1 rows).
+template <int NS>
+struct Shape {
+  static constexpr int KP = 4 * NS;
+  static constexpr int T1 = (KP + 15) / 16;
+  static constexpr int S2 = 4 * T1;  // stage-2 reduction steps (z rows 0 .. 16 T1)
+  static constexpr int T2 = (2 * KP + 1 + 15) / 16;
+  static constexpr int TT = (KP + 1 + 15) / 16;  // terms: [H; dsigma] row tiles
+};
+
+template <typename T>
+__device__ __forceinline__ double ldd(const T* p) {
+  return (double)*p;
+}
+
+// TRANS = false: the solve above.  TRANS = true: its transpose (the
+// vector-Jacobian product), which has the same two-stage shape:
+//   z_bar  = D_l (V^T dd - eta ((H V)^T dT + (dsigma^T V)^T dp))
+//   delta_bar = W^T z_bar, T_bar = dT - eta lam (W G)^T z_bar,
+//   lnps_bar = dp - eta lam w . z_bar
+// -- the tables are swapped by the host and eta / eta lam trade places.
+template <typename T, int NS, bool TRANS>
+__global__ __launch_bounds__(256) void solve_mfma_kernel(
+    const T* __restrict__ div, const T* __restrict__ temp, const T* __restrict__ lnps,
+    T* __restrict__ div_out, T* __restrict__ temp_out, T* __restrict__ lnps_out,
+    const double* __restrict__ t1a, const double* __restrict__ t1b,
+    const double* __restrict__ t2, const double* __restrict__ wvec,
+    const double* __restrict__ dvec, const double* __restrict__ lam, int K, int L,
+    int64_t cols, int64_t members, double eta) {
+  using S = Shape<NS>;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tiles_per_member = (cols + 15) / 16;
+  if (wave >= tiles_per_member * members) return;
+  const int64_t b = wave / tiles_per_member;
+  const int64_t c = (wave - b * tiles_per_member) * 16 + (lane & 15);
+  const int kg = lane >> 4;
+  const bool cok = c < cols;
+  const int64_t cc = cok ? c : 0;
+  const int64_t base = b * (int64_t)K * cols + cc;
+  const double lam_l = lam[(int)(cc % L)];
+  const double lp = ldd(lnps + b * cols + cc);
+  // stage-1 B fragments: delta and T rows 4 s + kg (zero beyond K)
+  double xd[NS], xt[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int k = 4 * s + kg;
+    const bool ok = k < K;
+    const double dv = ldd(div + base + (int64_t)(ok ? k : 0) * cols);
+    const double tv = ldd(temp + base + (int64_t)(ok ? k : 0) * cols);
+    xd[s] = ok ? dv : 0.0;
+    xt[s] = ok ? tv : 0.0;
+  }
+  doublex4 ca[S::T1], cb[S::T1];
+#pragma unroll
+  for (int it = 0; it < S::T1; ++it) {
+    ca[it] = doublex4{0.0, 0.0, 0.0, 0.0};
+    cb[it] = ca[it];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      ca[it] = mma64(t1a[(it * NS + s) * 64 + lane], xd[s], ca[it]);
+      cb[it] = mma64(t1b[(it * NS + s) * 64 + lane], xt[s], cb[it]);
+    }
+  }
+  // z = (C_a - eta lam (C_b + w lnps)) / (1 - eta^2 lam d), rows >= K vanish
+  double z[S::S2];
+#pragma unroll
+  for (int it = 0; it < S::T1; ++it) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = 16 * it + 4 * i + kg;
+      const double c1 = TRANS ? eta : eta * lam_l;
+      const double rhs = ca[it][i] - c1 * (cb[it][i] + wvec[row] * lp);
+      z[4 * it + i] = rhs / (1.0 - eta * eta * lam_l * dvec[row]);
+    }
+  }
+  // stage 2, one output row tile at a time
+  const double c2 = TRANS ? eta * lam_l : eta;
+  double sacc = 0.0;
+#pragma unroll
+  for (int gt = 0; gt < S::T2; ++gt) {
+    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < S::S2; ++s) acc = mma64(t2[(gt * S::S2 + s) * 64 + lane], z[s], acc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = 16 * gt + 4 * i + kg;
+      if (row < S::KP) {
+        if (cok && row < K) div_out[base + (int64_t)row * cols] = (T)acc[i];
+      } else if (row < 2 * S::KP) {
+        const int q = row - S::KP;  // = 4 s + kg with s = 4 gt + i - NS
+        const int s = 4 * gt + i - NS;
+        if (cok && q < K && s >= 0 && s < NS)
+          temp_out[base + (int64_t)q * cols] = (T)(xt[s] - c2 * acc[i]);
+      } else if (row == 2 * S::KP) {
+        sacc = acc[i];
+      }
+    }
+  }
+  // the lnps row lives in one k-group; route it to the lanes of kg = 0
+  if (kg == ((2 * S::KP) & 3) && cok) lnps_out[b * cols + cc] = (T)(lp - c2 * sacc);
+}
+
+// TRANS = false: the implicit terms.  TRANS = true: their transpose
+//   delta_bar = -(H^T dT + dsigma dp),  T_bar = -lam G^T dd,
+//   lnps_bar = -lam (R T_ref) . dd
+// with tg = H^T, th = [G^T; (R T_ref)^T], v1 = dsigma (host-side swap).
+template <typename T, int NS, bool TRANS>
+__global__ __launch_bounds__(256) void terms_mfma_kernel(
+    const T* __restrict__ div, const T* __restrict__ temp, const T* __restrict__ lnps,
+    T* __restrict__ div_out, T* __restrict__ temp_out, T* __restrict__ lnps_out,
+    const double* __restrict__ tg, const double* __restrict__ th,
+    const double* __restrict__ rtref, const double* __restrict__ lam, int K, int L,
+    int64_t cols, int64_t members) {
+  using S = Shape<NS>;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t tiles_per_member = (cols + 15) / 16;
+  if (wave >= tiles_per_member * members) return;
+  const int64_t b = wave / tiles_per_member;
+  const int64_t c = (wave - b * tiles_per_member) * 16 + (lane & 15);
+  const int kg = lane >> 4;
+  const bool cok = c < cols;
+  const int64_t cc = cok ? c : 0;
+  const int64_t base = b * (int64_t)K * cols + cc;
+  const double lam_l = lam[(int)(cc % L)];
+  const double lp = ldd(lnps + b * cols + cc);
+  double xd[NS], xt[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int k = 4 * s + kg;
+    const bool ok = k < K;
+    const double dv = ldd(div + base + (int64_t)(ok ? k : 0) * cols);
+    const double tv = ldd(temp + base + (int64_t)(ok ? k : 0) * cols);
+    xd[s] = ok ? dv : 0.0;
+    xt[s] = ok ? tv : 0.0;
+  }
+  // divergence tendency: -lam (G T + R T_ref lnps)
+#pragma unroll
+  for (int it = 0; it < S::T1; ++it) {
+    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc = mma64(tg[(it * NS + s) * 64 + lane], xt[s], acc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = 16 * it + 4 * i + kg;
+      const double s1 = TRANS ? 1.0 : lam_l;
+      if (cok && row < K)
+        div_out[base + (int64_t)row * cols] = (T)(-s1 * (acc[i] + rtref[row] * lp));
+    }
+  }
+  // temperature and log-pressure tendencies: -[H; dsigma] delta
+  const double s2 = TRANS ? lam_l : 1.0;
+  double sacc = 0.0;
+#pragma unroll
+  for (int it = 0; it < S::TT; ++it) {
+    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc = mma64(th[(it * NS + s) * 64 + lane], xd[s], acc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = 16 * it + 4 * i + kg;
+      if (row < K) {
+        if (cok) temp_out[base + (int64_t)row * cols] = (T)(-s2 * acc[i]);
+      } else if (row == S::KP) {
+        sacc = acc[i];
+      }
+    }
+  }
+  if (kg == (S::KP & 3) && cok) lnps_out[b * cols + cc] = (T)(-s2 * sacc);
+}
+
+}  // namespace
+
+// Host-side construction of the fragment tables.  A table for a (rows x kdim)
+// matrix M in tiles of 16 rows and reduction steps of 4:
+//   tab[(tile * steps + s) * 64 + lane] = M[16 tile + lane % 16][4 s + lane / 16].
+static void fragment_table(std::vector<double>& tab, int tiles, int steps,
+                           const std::function<double(int, int)>& M) {
+  tab.assign((size_t)tiles * steps * 64, 0.0);
+  for (int t = 0; t < tiles; ++t)
+    for (int s = 0; s < steps; ++s)
+      for (int lane = 0; lane < 64; ++lane)
+        tab[((size_t)t * steps + s) * 64 + lane] = M(16 * t + lane % 16, 4 * s + lane / 16);
+}
+
+int implicit_mfma_steps(int K) { return (K + 3) / 4; }
+
+// Builds the tables for NS = ceil(K / 4).  Inputs are row-major K x K FP64.
+int implicit_mfma_tables(int K, const double* G, const double* H, const double* tref,
+                         const double* dsigma, const double* V, const double* Vinv,
+                         const double* d, double r_gas, std::vector<double>* out) {
+  const int NS = implicit_mfma_steps(K), KP = 4 * NS;
+  const int T1 = (KP + 15) / 16, S2 = 4 * T1, T2 = (2 * KP + 1 + 15) / 16;
+  const int TT = (KP + 1 + 15) / 16;
+  auto at = [&](const double* A, int r, int c) { return A[(size_t)r * K + c]; };
+  std::vector<double> WG((size_t)K * K, 0.0), HV((size_t)K * K, 0.0), w(K, 0.0), sV(K, 0.0);
+  if (V && Vinv) {
+    for (int r = 0; r < K; ++r)
+      for (int c = 0; c < K; ++c) {
+        double a = 0.0, h = 0.0;
+        for (int k = 0; k < K; ++k) {
+          a += at(Vinv, r, k) * at(G, k, c);
+          h += at(H, r, k) * at(V, k, c);
+        }
+        WG[(size_t)r * K + c] = a;
+        HV[(size_t)r * K + c] = h;
+      }
+    for (int r = 0; r < K; ++r) {
+      double a = 0.0, s = 0.0;
+      for (int k = 0; k < K; ++k) {
+        a += at(Vinv, r, k) * r_gas * tref[k];
+        s += dsigma[k] * at(V, k, r);
+      }
+      w[r] = a;
+      sV[r] = s;
+    }
+  }
+  auto in = [&](int r, int c) { return r < K && c < K; };
+  auto vec16 = [&](std::vector<double>& v, const std::function<double(int)>& f) {
+    v.assign(16 * T1, 0.0);
+    for (int r = 0; r < K; ++r) v[r] = f(r);
+  };
+  const bool modes = V && Vinv;
+  // forward: [0] t1a = W, [1] t1b = W G, [2] t2 = [V; H V; dsigma^T V], [3] w,
+  // [4] d, [5] tg = G, [6] th = [H; dsigma^T], [7] R T_ref
+  fragment_table(out[0], T1, NS, [&](int r, int c) { return modes && in(r, c) ? at(Vinv, r, c) : 0.0; });
+  fragment_table(out[1], T1, NS, [&](int r, int c) { return in(r, c) ? WG[(size_t)r * K + c] : 0.0; });
+  fragment_table(out[2], T2, S2, [&](int r, int c) {
+    if (!modes || c >= K) return 0.0;
+    if (r < KP) return r < K ? at(V, r, c) : 0.0;
+    if (r < 2 * KP) return r - KP < K ? HV[(size_t)(r - KP) * K + c] : 0.0;
+    return r == 2 * KP ? sV[c] : 0.0;
+  });
+  vec16(out[3], [&](int r) { return w[r]; });
+  vec16(out[4], [&](int r) { return d ? d[r] : 0.0; });
+  fragment_table(out[5], T1, NS, [&](int r, int c) { return in(r, c) ? at(G, r, c) : 0.0; });
+  fragment_table(out[6], TT, NS, [&](int r, int c) {
+    if (c >= K) return 0.0;
+    if (r < K) return at(H, r, c);
+    return r == KP ? dsigma[c] : 0.0;
+  });
+  vec16(out[7], [&](int r) { return r_gas * tref[r]; });
+  // transposes: [8] V^T, [9] (H V)^T, [10] [W^T; (W G)^T; w^T], [11] dsigma^T V,
+  // [12] H^T, [13] [G^T; (R T_ref)^T], [14] dsigma
+  fragment_table(out[8], T1, NS, [&](int r, int c) { return modes && in(r, c) ? at(V, c, r) : 0.0; });
+  fragment_table(out[9], T1, NS, [&](int r, int c) { return in(r, c) ? HV[(size_t)c * K + r] : 0.0; });
+  fragment_table(out[10], T2, S2, [&](int r, int c) {
+    if (!modes || c >= K) return 0.0;
+    if (r < KP) return r < K ? at(Vinv, c, r) : 0.0;
+    if (r < 2 * KP) return r - KP < K ? WG[(size_t)c * K + (r - KP)] : 0.0;
+    return r == 2 * KP ? w[c] : 0.0;
+  });
+  vec16(out[11], [&](int r) { return sV[r]; });
+  fragment_table(out[12], T1, NS, [&](int r, int c) { return in(r, c) ? at(H, c, r) : 0.0; });
+  fragment_table(out[13], TT, NS, [&](int r, int c) {
+    if (c >= K) return 0.0;
+    if (r < K) return at(G, c, r);
+    return r == KP ? r_gas * tref[c] : 0.0;
+  });
+  vec16(out[14], [&](int r) { return dsigma[r]; });
+  return DINO_OK;
+}
+
+template <typename T, int NS>
+static int launch_ns(bool solve, bool adjoint, const dino_implicit* f, const void* div,
+                     const void* temp, const void* lnps, void* d_o, void* t_o, void* l_o,
+                     int64_t members, hipStream_t s) {
+  const int64_t cols = (int64_t)f->R * f->L;
+  const int64_t waves = (cols + 15) / 16 * members;
+  if (waves == 0) return DINO_OK;
+  const unsigned blocks = (unsigned)((waves + 3) / 4);
+  const T* x0 = (const T*)div;
+  const T* x1 = (const T*)temp;
+  const T* x2 = (const T*)lnps;
+  T* y0 = (T*)d_o;
+  T* y1 = (T*)t_o;
+  T* y2 = (T*)l_o;
+  double* const* t = f->mtab;
+  if (solve && !adjoint) {
+    solve_mfma_kernel<T, NS, false><<<blocks, 256, 0, s>>>(
+        x0, x1, x2, y0, y1, y2, t[0], t[1], t[2], t[3], t[4], f->lam, f->K, f->L, cols,
+        members, f->eta);
+  } else if (solve) {
+    solve_mfma_kernel<T, NS, true><<<blocks, 256, 0, s>>>(
+        x0, x1, x2, y0, y1, y2, t[8], t[9], t[10], t[11], t[4], f->lam, f->K, f->L, cols,
+        members, f->eta);
+  } else if (!adjoint) {
+    terms_mfma_kernel<T, NS, false><<<blocks, 256, 0, s>>>(
+        x0, x1, x2, y0, y1, y2, t[5], t[6], t[7], f->lam, f->K, f->L, cols, members);
+  } else {
+    terms_mfma_kernel<T, NS, true><<<blocks, 256, 0, s>>>(
+        x0, x1, x2, y0, y1, y2, t[12], t[13], t[14], f->lam, f->K, f->L, cols, members);
+  }
+  DINO_LAUNCH_CHECK("implicit mfma kernel");
+  return DINO_OK;
+}
+
+template <typename T>
+int implicit_mfma_dispatch(bool solve, bool adjoint, const dino_implicit* f, const void* div,
+                           const void* temp, const void* lnps, void* d_o, void* t_o, void* l_o,
+                           int64_t members, hipStream_t s) {
+  switch (implicit_mfma_steps(f->K)) {
+#define NS_CASE(n) \
+  case n:          \
+    return launch_ns<T, n>(solve, adjoint, f, div, temp, lnps, d_o, t_o, l_o, members, s);
+    NS_CASE(1) NS_CASE(2) NS_CASE(3) NS_CASE(4) NS_CASE(5) NS_CASE(6) NS_CASE(7) NS_CASE(8)
+    NS_CASE(9) NS_CASE(10) NS_CASE(11) NS_CASE(12) NS_CASE(13) NS_CASE(14) NS_CASE(15)
+    NS_CASE(16)
+#undef NS_CASE
+    default:
+      set_error("implicit solve supports up to 64 levels, got %d", f->K);
+      return DINO_EINVAL;
+  }
+}
+
+template int implicit_mfma_dispatch<float>(bool, bool, const dino_implicit*, const void*,
+                                           const void*, const void*, void*, void*, void*,
+                                           int64_t, hipStream_t);
+template int implicit_mfma_dispatch<double>(bool, bool, const dino_implicit*, const void*,
+                                            const void*, const void*, void*, void*, void*,
+                                            int64_t, hipStream_t);
+
+}  // namespace dino

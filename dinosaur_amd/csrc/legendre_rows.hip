@@ -82,7 +82,12 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
   constexpr int KT = 2 * KS;  // 16-node groups
   constexpr int TF4 = tile_f4(KS);
   constexpr int PER = TF4 / 256;
-  __shared__ floatx4 pbuf[2][TF4];
+  // One LDS region: first the staging of the folded rows (64 rows, pitch
+  // padded by 4 floats against bank conflicts), then the table double buffer.
+  constexpr int PITCH = 32 * KS + 4;
+  constexpr int STG4 = kRowsPerBlock * PITCH / 4;
+  constexpr int LDS4 = (2 * TF4 > STG4) ? 2 * TF4 : STG4;
+  __shared__ floatx4 lds[LDS4];
   const int m = blockIdx.y;
   const int nrows = (m == 0) ? B : 2 * B;
   const int row0 = blockIdx.x * kRowsPerBlock;
@@ -102,39 +107,74 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
   // Folded rows: X0 pairs with the even-d degrees of a tile (relative
   // parity (m & 1)), X1 with the odd-d ones.  For odd m the roles of the
   // sum and the difference swap.
-  // Loads are unconditional from clamped addresses (invalid slots are zeroed
-  // afterwards by selects), so all 2 KT loads of a lane issue back to back.
+  const float sgn = (m & 1) ? -1.f : 1.f;
+  const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
   float4 x0[KT], x1[KT];
-  {
+  if constexpr (VEC) {
+    // Coalesced loads: item (row rho of the wave's strip, 4-node chunk c);
+    // consecutive lanes take consecutive chunks of a row, northern half
+    // ascending and its mirror image descending.  Addresses are clamped and
+    // invalid items zeroed by selects, so all loads issue back to back.
+    const int H4 = Jh / 4;
+    floatx4 u[KT], d[KT];
+#pragma unroll
+    for (int it = 0; it < KT; ++it) {
+      const int idx = lane + 64 * it;
+      const int rho = idx / H4, c = idx - rho * H4;
+      const int row = row0 + wave * 16 + rho;
+      const bool ok = rho < 16 && row < nrows;
+      int b = 0, r = 0;
+      if (ok) row_field(m, row, &b, &r);
+      const float* f = fourier + ((int64_t)b * Rrows + r) * J;
+      const floatx4 uu = *reinterpret_cast<const floatx4*>(f + (ok ? J - Jh + 4 * c : 0));
+      const floatx4 dr = *reinterpret_cast<const floatx4*>(f + (ok ? Jh - 4 - 4 * c : 0));
+      u[it] = ok ? uu : zero;
+      d[it] = ok ? floatx4{dr[3], dr[2], dr[1], dr[0]} : zero;
+    }
+    // Transpose through LDS into A-operand fragments, X0 first, then X1.
+    float* stg = reinterpret_cast<float*>(lds);
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+      const float sp = pass == 0 ? sgn : -sgn;
+#pragma unroll
+      for (int it = 0; it < KT; ++it) {
+        const int idx = lane + 64 * it;
+        const int rho = idx / H4, c = idx - rho * H4;
+        if (rho < 16)
+          *reinterpret_cast<floatx4*>(stg + (wave * 16 + rho) * PITCH + 4 * c) = u[it] + sp * d[it];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < KT; ++t) {
+        const int jj = 16 * t + 4 * g;
+        const floatx4 v = *reinterpret_cast<const floatx4*>(stg + (wave * 16 + c16) * PITCH + jj);
+        const floatx4 w = jj < Jh ? v : zero;
+        if (pass == 0) x0[t] = make_float4(w[0], w[1], w[2], w[3]);
+        else x1[t] = make_float4(w[0], w[1], w[2], w[3]);
+      }
+      __syncthreads();
+    }
+  } else {
+    // Generic shapes: gather the fragments directly.
     const int row = row0 + wave * 16 + c16;
     const bool rok = row < nrows;
-    const float sgn = (m & 1) ? -1.f : 1.f;
     int b = 0, r = 0;
     if (rok) row_field(m, row, &b, &r);
     const float* f = fourier + ((int64_t)b * Rrows + r) * J;
-    const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < KT; ++t) {
       const int jj = 16 * t + 4 * g;
       floatx4 u, d;
-      if constexpr (VEC) {
-        const bool ok = rok && jj < Jh;
-        const floatx4 uu = *reinterpret_cast<const floatx4*>(f + (ok ? J - Jh + jj : 0));
-        const floatx4 dr = *reinterpret_cast<const floatx4*>(f + (ok ? Jh - 4 - jj : 0));
-        u = ok ? uu : zero;
-        d = ok ? floatx4{dr[3], dr[2], dr[1], dr[0]} : zero;
-      } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int q = jj + e;
-          const bool ok = rok && q < Jh;
-          // odd J: node jj = 0 is the equator, counted once
-          const bool okd = ok && !((J & 1) && q == 0);
-          const float uu = f[ok ? J - Jh + q : 0];
-          const float dd = f[okd ? Jh - 1 - q : 0];
-          u[e] = ok ? uu : 0.f;
-          d[e] = okd ? dd : 0.f;
-        }
+      for (int e = 0; e < 4; ++e) {
+        const int q = jj + e;
+        const bool ok = rok && q < Jh;
+        // odd J: node jj = 0 is the equator, counted once
+        const bool okd = ok && !((J & 1) && q == 0);
+        const float uu = f[ok ? J - Jh + q : 0];
+        const float dd = f[okd ? Jh - 1 - q : 0];
+        u[e] = ok ? uu : 0.f;
+        d[e] = okd ? dd : 0.f;
       }
       x0[t] = make_float4(u[0] + sgn * d[0], u[1] + sgn * d[1], u[2] + sgn * d[2],
                           u[3] + sgn * d[3]);
@@ -192,7 +232,7 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
   }
 
   for (int tt = 0; tt < ntile; ++tt) {
-    floatx4* buf = pbuf[tt & 1];
+    floatx4* buf = lds + (tt & 1) * TF4;
 #pragma unroll
     for (int i = 0; i < PER; ++i) buf[tid + 256 * i] = st[i];
     __syncthreads();

@@ -95,6 +95,10 @@ class State:
         return None
     return buf
 
+  def as_flat(self):
+    """This state backed by one (fields, R, L) buffer (see to_flat)."""
+    return to_flat(self)
+
   def tree_map_arrays(self, fn):
     kw = {f: fn(getattr(self, f)) for f in self._ARRAYS}
     kw['tracers'] = {k: fn(v) for k, v in self.tracers.items()}
@@ -381,62 +385,32 @@ class _ImplicitOperator:
 
 
 class _ImplicitSolve(torch.autograd.Function):
-  """(I - eta G)^{-1} on (div, temp, lnps); the adjoint applies the transposed
-  inverse, computed with the same kernel on the transposed factor."""
+  """(I - eta G)^{-1} on (div, temp, lnps); the backward is the transposed
+  solve kernel (dino_implicit_solve_adjoint)."""
 
   @staticmethod
-  def forward(ctx, div, temp, lnps, op, op_t, eta):
-    ctx.op_t, ctx.eta = op_t, eta
+  def forward(ctx, div, temp, lnps, op, eta):
+    ctx.op, ctx.eta = op, eta
     return tuple(op.apply('dino_implicit_solve', eta, div, temp, lnps))
 
   @staticmethod
   def backward(ctx, gd, gt, gl):
-    return tuple(ctx.op_t.apply_transpose(ctx.eta, gd, gt, gl)) + (None, None, None)
+    return tuple(ctx.op.apply('dino_implicit_solve_adjoint', ctx.eta, gd, gt, gl)) + (
+        None, None)
 
 
 class _ImplicitTerms(torch.autograd.Function):
+  """Implicit tendencies; the backward is dino_implicit_terms_adjoint."""
 
   @staticmethod
   def forward(ctx, div, temp, lnps, op, eta):
-    ctx.op = op
+    ctx.op, ctx.eta = op, eta
     return tuple(op.apply('dino_implicit_terms', eta, div, temp, lnps))
 
   @staticmethod
   def backward(ctx, gd, gt, gl):
-    return tuple(ctx.op.terms_transpose(gd, gt, gl)) + (None, None)
-
-
-def _implicit_transpose_ops(op: _ImplicitOperator):
-  """Adjoints of the implicit maps in torch FP64 level algebra.
-
-  terms: y_d = -lam (G T + R tref lnps), y_t = -H d, y_l = -ds . d
-  adjoint: d_bar = -H^T y_t - ds y_l ; T_bar = -G^T (lam y_d) ;
-           lnps_bar = -R tref . (lam y_d)
-  """
-
-  def terms_transpose(gd, gt, gl):
-    dev, dt = gd.device, gd.dtype
-    lam = torch.as_tensor(op.lam, device=dev, dtype=torch.float64)
-    g = torch.as_tensor(op.G, device=dev)
-    h = torch.as_tensor(op.H, device=dev)
-    tref = torch.as_tensor(op.tref, device=dev)
-    ds = torch.as_tensor(op.dsig, device=dev)
-    ld = gd.double() * lam
-    d_bar = -torch.einsum('hg,...hml->...gml', h, gt.double()) - ds[:, None, None] * gl.double()
-    t_bar = -torch.einsum('hg,...hml->...gml', g, ld)
-    l_bar = -op.r_gas * torch.einsum('h,...hml->...ml', tref, ld).unsqueeze(-3)
-    return d_bar.to(dt), t_bar.to(dt), l_bar.to(dt)
-
-  def apply_transpose(eta, gd, gt, gl):
-    # (M_l^{-1})^T = (M_l^T)^{-1}; solve with the dense FP64 per-l matrices.
-    dev, dt = gd.device, gd.dtype
-    mats = op.dense_inverse_t(eta, dev)  # (L, 2K+1, 2K+1) = inv(M_l)^T
-    k = op.K
-    x = torch.cat([gd.double(), gt.double(), gl.double()], dim=-3)  # (..., 2K+1, R, L)
-    y = torch.einsum('lgh,...hml->...gml', mats, x)
-    return y[..., :k, :, :].to(dt), y[..., k:2 * k, :, :].to(dt), y[..., 2 * k:, :, :].to(dt)
-
-  return terms_transpose, apply_transpose
+    return tuple(ctx.op.apply('dino_implicit_terms_adjoint', ctx.eta, gd, gt, gl)) + (
+        None, None)
 
 
 # ------------------------------------------------------------------ fused nodal terms
@@ -564,41 +538,33 @@ def _state_views(buf, members, K, nt):
 
 class _ImplicitFlat(torch.autograd.Function):
   """Implicit terms (solve=False) or (I - eta G)^{-1} (solve=True) on a flat
-  state buffer; FP64 GPU kernels."""
+  state buffer; FP64 GPU kernels, transposed kernels as backward."""
 
   @staticmethod
-  def forward(ctx, buf, op, eta, members, nt, solve):
-    K = op.K
-    buf = buf.contiguous()
-    out = torch.zeros_like(buf) if not solve else torch.empty_like(buf)
+  def _run(buf, op, eta, members, nt, solve, adjoint):
+    K, E = op.K, members
+    out = torch.empty_like(buf)
     d, t, l = _state_views(buf, members, K, nt)
     od, ot, ol = _state_views(out, members, K, nt)
-    if solve:
-      E = members
-      out[:E * K].copy_(buf[:E * K])                  # vorticity passes through
-      out[3 * E * K + E:].copy_(buf[3 * E * K + E:])  # tracers pass through
-    op.apply_into('dino_implicit_solve' if solve else 'dino_implicit_terms', eta, members,
-                  d, t, l, od, ot, ol)
-    ctx.op, ctx.eta, ctx.members, ctx.nt, ctx.solve = op, eta, members, nt, solve
+    if solve:  # vorticity and tracers pass through
+      out[:E * K].copy_(buf[:E * K])
+      out[3 * E * K + E:].copy_(buf[3 * E * K + E:])
+    else:  # they have no implicit tendency
+      out[:E * K].zero_()
+      out[3 * E * K + E:].zero_()
+    name = 'dino_implicit_solve' if solve else 'dino_implicit_terms'
+    op.apply_into(name + ('_adjoint' if adjoint else ''), eta, members, d, t, l, od, ot, ol)
     return out
 
   @staticmethod
+  def forward(ctx, buf, op, eta, members, nt, solve):
+    ctx.op, ctx.eta, ctx.members, ctx.nt, ctx.solve = op, eta, members, nt, solve
+    return _ImplicitFlat._run(buf.contiguous(), op, eta, members, nt, solve, False)
+
+  @staticmethod
   def backward(ctx, g):
-    op, E, K = ctx.op, ctx.members, ctx.op.K
-    g = g.contiguous()
-    gd, gt, gl = _state_views(g, E, K, ctx.nt)
-    R_, L_ = g.shape[1:]
-    shp = lambda t, kk: t.reshape(E, kk, R_, L_)
-    if ctx.solve:
-      gin = g.clone()  # vorticity and tracers pass through
-      a, b, c = op.apply_transpose(ctx.eta, shp(gd, K), shp(gt, K), shp(gl, 1))
-    else:
-      gin = torch.zeros_like(g)
-      a, b, c = op.terms_transpose(shp(gd, K), shp(gt, K), shp(gl, 1))
-    vd, vt, vl = _state_views(gin, E, K, ctx.nt)
-    vd.copy_(a.reshape(vd.shape))
-    vt.copy_(b.reshape(vt.shape))
-    vl.copy_(c.reshape(vl.shape))
+    gin = _ImplicitFlat._run(g.contiguous(), ctx.op, ctx.eta, ctx.members, ctx.nt, ctx.solve,
+                             True)
     return gin, None, None, None, None, None
 
 
@@ -641,22 +607,8 @@ class PrimitiveEquationsSigma:
   # ---- static data on the device
   @functools.cached_property
   def _implicit(self) -> _ImplicitOperator:
-    op = _ImplicitOperator(self.coords, self.reference_temperature,
-                           self.physics_specs.kappa, self.physics_specs.R)
-    terms_t, apply_t = _implicit_transpose_ops(op)
-    op.terms_transpose = terms_t
-    op.apply_transpose = apply_t
-    cache = {}
-
-    def dense_inverse_t(eta, dev):
-      key = (float(eta), dev)
-      if key not in cache:
-        m = _get_implicit_term_matrix_sigma(eta, self.coords, self.reference_temperature,
-                                            self.physics_specs.kappa, self.physics_specs.R)
-        cache[key] = torch.as_tensor(np.linalg.inv(m).transpose(0, 2, 1).copy(), device=dev)
-      return cache[key]
-    op.dense_inverse_t = dense_inverse_t
-    return op
+    return _ImplicitOperator(self.coords, self.reference_temperature,
+                             self.physics_specs.kappa, self.physics_specs.R)
 
   def _c(self, name, like: torch.Tensor):
     key = (name, like.device, like.dtype)
@@ -989,7 +941,7 @@ class PrimitiveEquationsSigma:
       out = _ImplicitFlat.apply(buf, op, float(step_size), members, len(names), True)
       return State.from_flat(out, lead, K, names, sim_time=state.sim_time)
     d, t, l = _ImplicitSolve.apply(state.divergence, state.temperature_variation,
-                                   state.log_surface_pressure, op, op, float(step_size))
+                                   state.log_surface_pressure, op, float(step_size))
     return State(state.vorticity, d, t, l, dict(state.tracers), sim_time=state.sim_time)
 
   # ---- reference helper methods kept for API compatibility

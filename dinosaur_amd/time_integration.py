@@ -7,7 +7,9 @@ in the backward pass, the analogue of ``nested_checkpoint_scan``).
 """
 from __future__ import annotations
 
+import ctypes
 import dataclasses
+import functools
 import math
 from typing import Any, Callable, Sequence
 
@@ -114,7 +116,83 @@ def _weighted_sum(terms):
   return acc
 
 
+def _flat_of(x):
+  """(buffer, layout) of a flat-backed state, or None."""
+  fb = getattr(x, 'flat_buffer', None)
+  buf = fb() if fb is not None else None
+  return None if buf is None else (buf, x._flat[1:])
+
+
+class _ImexStage(torch.autograd.Function):
+  """y0 + dt * sum(a f) + dt * sum(b g) over flat buffers in one pass
+  (dino_imex_stage); the backward writes every term's cotangent in one pass."""
+
+  @staticmethod
+  def forward(ctx, n_ex, coefs, dt, y0, *xs):
+    from . import _lib
+    y0 = y0.contiguous()
+    xs = [x.contiguous() for x in xs]
+    out = torch.empty_like(y0)
+    n_im = len(xs) - n_ex
+    P = lambda ts: (ctypes.c_void_p * max(len(ts), 1))(*[t.data_ptr() for t in ts])
+    D = lambda cs: (ctypes.c_double * max(len(cs), 1))(*cs)
+    code = _lib.DINO_F64 if y0.dtype == torch.float64 else _lib.DINO_F32
+    stream = ctypes.c_void_p(torch.cuda.current_stream(y0.device).cuda_stream)
+    with torch.cuda.device(y0.device):
+      rc = _lib.lib().dino_imex_stage(
+          ctypes.c_void_p(y0.data_ptr()), n_ex, P(xs[:n_ex]), D(coefs[:n_ex]), n_im,
+          P(xs[n_ex:]), D(coefs[n_ex:]), float(dt), ctypes.c_void_p(out.data_ptr()),
+          y0.numel(), code, stream)
+    _lib.check(rc, 'dino_imex_stage')
+    ctx.n_ex, ctx.coefs, ctx.dt, ctx.n = n_ex, coefs, dt, len(xs)
+    return out
+
+  @staticmethod
+  def backward(ctx, g):
+    from . import _lib
+    g = g.contiguous()
+    n_ex, n_im = ctx.n_ex, ctx.n - ctx.n_ex
+    grads = [torch.empty_like(g) for _ in range(ctx.n)]
+    P = lambda ts: (ctypes.c_void_p * max(len(ts), 1))(*[t.data_ptr() for t in ts])
+    D = lambda cs: (ctypes.c_double * max(len(cs), 1))(*cs)
+    code = _lib.DINO_F64 if g.dtype == torch.float64 else _lib.DINO_F32
+    stream = ctypes.c_void_p(torch.cuda.current_stream(g.device).cuda_stream)
+    with torch.cuda.device(g.device):
+      rc = _lib.lib().dino_imex_stage_adjoint(
+          ctypes.c_void_p(g.data_ptr()), n_ex, D(ctx.coefs[:n_ex]), n_im,
+          D(ctx.coefs[n_ex:]), float(ctx.dt), P(grads[:n_ex]), P(grads[n_ex:]), g.numel(),
+          code, stream)
+    _lib.check(rc, 'dino_imex_stage_adjoint')
+    return (None, None, None, g) + tuple(grads)
+
+
+def _stage_time(y0, dt, groups):
+  """sim_time of the stage value, by the rules of elementwise state algebra
+  (None as soon as an operand has none)."""
+  st = y0.sim_time
+  for terms in groups:
+    if not terms:
+      continue
+    s = None
+    for a, x in terms:
+      if x.sim_time is None:
+        s = None
+        break
+      t = a * x.sim_time
+      s = t if s is None else s + t
+    st = None if (st is None or s is None) else st + dt * s
+  return st
+
+
 def _stage(y0, dt, ex, im):
+  """y0 + dt * sum(a_ex f) + dt * sum(a_im g) (time_integration.py:351-370)."""
+  base = _flat_of(y0)
+  if base is not None and base[0].is_cuda and 0 < len(ex) + len(im) and len(ex) <= 4 and len(im) <= 4:
+    flats = [_flat_of(x) for _, x in ex + im]
+    if all(f is not None and f[1] == base[1] for f in flats):
+      coefs = [float(a) for a, _ in ex + im]
+      buf = _ImexStage.apply(len(ex), coefs, float(dt), base[0], *[f[0] for f in flats])
+      return type(y0).from_flat(buf, *base[1], sim_time=_stage_time(y0, dt, (ex, im)))
   out = y0
   for terms in (ex, im):
     if terms:
@@ -130,6 +208,10 @@ def imex_runge_kutta(tableau: ImExButcherTableau, equation, time_step: float):
   n = len(b_ex)
 
   def step_fn(y0):
+    # one flat buffer per state: every stage combination is a single pass
+    as_flat = getattr(y0, 'as_flat', None)
+    if as_flat is not None:
+      y0 = as_flat()
     f = [None] * n
     g = [None] * n
     f[0] = F(y0)
@@ -248,20 +330,28 @@ def checkpointed_rollout(step_fn, x, steps: int, checkpoint_every: int):
   role of nested_checkpoint_scan, time_integration.py:573-637)."""
   if steps % checkpoint_every:
     raise ValueError(f'inconsistent steps={steps} and checkpoint_every={checkpoint_every}')
+  as_flat = getattr(x, 'as_flat', None)
+  if as_flat is not None:
+    x = as_flat()
   leaves, rebuild = _flatten_state(x)
+  structure = {}
 
-  def segment(*ls):
-    y = rebuild(list(ls))
+  def segment(rebuild_in, *ls):
+    # rebuild_in is bound per segment, so the recomputation in the backward
+    # pass rebuilds the same structure as the forward call did
+    y = rebuild_in(list(ls))
     for _ in range(checkpoint_every):
       y = step_fn(y)
-    out, _ = _flatten_state(y)
+    out, structure['rebuild'] = _flatten_state(y)
     return tuple(out)
 
   for _ in range(steps // checkpoint_every):
+    fn = functools.partial(segment, rebuild)
     if torch.is_grad_enabled() and any(t.requires_grad for t in leaves):
-      leaves = list(torch.utils.checkpoint.checkpoint(segment, *leaves, use_reentrant=False))
+      leaves = list(torch.utils.checkpoint.checkpoint(fn, *leaves, use_reentrant=False))
     else:
-      leaves = list(segment(*leaves))
+      leaves = list(fn(*leaves))
+    rebuild = structure['rebuild']
   return rebuild(leaves)
 
 

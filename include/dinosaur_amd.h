@@ -162,6 +162,43 @@ int dino_implicit_terms(const dino_implicit* f, const void* div,
                         void* temp_out, void* lnps_out, int64_t members,
                         int dtype, void* stream);
 
+/* Vector-Jacobian products of dino_implicit_solve and dino_implicit_terms:
+ * given the cotangents of (div_out, temp_out, lnps_out), write the
+ * cotangents of (div, temp, lnps).  They stand in for what jax.vjp derives
+ * from PrimitiveEquationsSigma.implicit_inverse and implicit_terms
+ * (dinosaur/primitive_equations.py:1196-1458) when the reference is
+ * differentiated; same layouts and no-alias rule as the forward calls. */
+int dino_implicit_solve_adjoint(const dino_implicit* f, const void* div_cot,
+                                const void* temp_cot, const void* lnps_cot,
+                                void* div_out, void* temp_out, void* lnps_out,
+                                int64_t members, int dtype, void* stream);
+int dino_implicit_terms_adjoint(const dino_implicit* f, const void* div_cot,
+                                const void* temp_cot, const void* lnps_cot,
+                                void* div_out, void* temp_out, void* lnps_out,
+                                int64_t members, int dtype, void* stream);
+
+/* IMEX Runge-Kutta stage combination on flat state buffers of n elements:
+ *   out = y0 + dt * (a[0] f[0] + ... + a[n_ex-1] f[n_ex-1])
+ *            + dt * (b[0] g[0] + ... + b[n_im-1] g[n_im-1])
+ * with the roundings of the reference's elementwise passes, in order
+ * (each product, partial sum and scaling rounded to dtype; coefficients
+ * rounded to dtype first).  Replaces the tree_map arithmetic of
+ * imex_runge_kutta (dinosaur/time_integration.py:351-370).  At most 4
+ * terms per group; a group may be empty.  out may alias y0. */
+int dino_imex_stage(const void* y0, int n_ex, const void* const* f,
+                    const double* a, int n_im, const void* const* g,
+                    const double* b, double dt, void* out, int64_t n,
+                    int dtype, void* stream);
+
+/* Cotangents of the f and g operands of dino_imex_stage given the output
+ * cotangent: f_cot[j] = a[j] * (dt * out_cot), g_cot[k] = b[k] * (dt *
+ * out_cot) (the y0 cotangent is out_cot itself).  What jax.vjp derives for
+ * the same arithmetic in the reference. */
+int dino_imex_stage_adjoint(const void* out_cot, int n_ex, const double* a,
+                            int n_im, const double* b, double dt,
+                            void* const* f_cot, void* const* g_cot, int64_t n,
+                            int dtype, void* stream);
+
 /* Fused nodal nonlinear terms of PrimitiveEquationsSigma.explicit_terms.
  * Replaces compute_diagnostic_state_sigma and the nodal parts of
  * curl_and_div_tendencies, kinetic_energy_tendency,

@@ -29,6 +29,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kRowsPerBlock = 64;
 constexpr int kTileDeg = 32;
+constexpr int kOutPitch = kTileDeg + 1;  // output staging row pitch (floats)
 
 __device__ __forceinline__ void row_field(int m, int q, int* b, int* r) {
   if (m == 0) {
@@ -72,9 +73,11 @@ __device__ __forceinline__ floatx4 ld4(const float4* p) {
   return *reinterpret_cast<const floatx4*>(p);
 }
 
-// VEC: J % 8 == 0, L % 4 == 0 and 16-byte aligned buffers (float4 row loads,
-// float2 / float4 stores); otherwise scalar accesses, any J and L.
-template <bool BF, int KS, bool VEC>
+// VIN: J % 8 == 0 and a 16-byte aligned Fourier buffer (coalesced float4 row
+// loads staged through LDS), otherwise a scalar gather for any J.
+// VOUT: L % 4 == 0 and a 16-byte aligned modal buffer (float2 / float4
+// stores), otherwise scalar stores for any L (TL grids have odd L).
+template <bool BF, int KS, bool VIN, bool VOUT>
 __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
     const float* __restrict__ fourier, float* __restrict__ modal,
     const float4* __restrict__ ftab, const int64_t* __restrict__ ftile_off, int B, int L,
@@ -88,6 +91,7 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
   constexpr int STG4 = kRowsPerBlock * PITCH / 4;
   constexpr int LDS4 = (2 * TF4 > STG4) ? 2 * TF4 : STG4;
   __shared__ floatx4 lds[LDS4];
+  __shared__ float ostage[VOUT ? 1 : 4 * 16 * kOutPitch];
   const int m = blockIdx.y;
   const int nrows = (m == 0) ? B : 2 * B;
   const int row0 = blockIdx.x * kRowsPerBlock;
@@ -110,7 +114,7 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
   const float sgn = (m & 1) ? -1.f : 1.f;
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
   float4 x0[KT], x1[KT];
-  if constexpr (VEC) {
+  if constexpr (VIN) {
     // Coalesced loads: item (row rho of the wave's strip, 4-node chunk c);
     // consecutive lanes take consecutive chunks of a row, northern half
     // ascending and its mirror image descending.  Addresses are clamped and
@@ -195,7 +199,7 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
 
   // Zero block: whole 32-degree blocks below m.
   if (lt0 > 0) {
-    if constexpr (VEC) {
+    if constexpr (VOUT) {
       const int n4 = lt0 / 4;
       for (int idx = tid; idx < kRowsPerBlock * n4; idx += 256) {
         const int q = idx / n4, c = idx - q * n4;
@@ -272,15 +276,32 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
         a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x1[t].w, p1[3], a1, 0, 0, 0);
       }
     }
-    const int l = lt0 + tt * kTileDeg + 2 * c16;
+    const int lt = lt0 + tt * kTileDeg;
+    if constexpr (VOUT) {
+      const int l = lt + 2 * c16;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (!orow[i]) continue;
-      if constexpr (VEC) {
-        if (l < L) *reinterpret_cast<float2*>(orow[i] + l) = make_float2(a0[i], a1[i]);
-      } else {
-        if (l < L) orow[i][l] = a0[i];
-        if (l + 1 < L) orow[i][l + 1] = a1[i];
+      for (int i = 0; i < 4; ++i)
+        if (orow[i] && l < L) *reinterpret_cast<float2*>(orow[i] + l) = make_float2(a0[i], a1[i]);
+    } else {
+      // Unaligned rows: transpose the wave's 16 x 32 tile through LDS so that
+      // each store instruction writes two contiguous 128-byte row segments.
+      float* os = ostage + wave * (16 * kOutPitch);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        os[(4 * g + i) * kOutPitch + 2 * c16] = a0[i];
+        os[(4 * g + i) * kOutPitch + 2 * c16 + 1] = a1[i];
+      }
+      __syncthreads();
+      const int col = lane & 31, l = lt + col;
+#pragma unroll
+      for (int rr = 0; rr < 8; ++rr) {
+        const int rl = 2 * rr + (lane >> 5);
+        const int row = row0 + wave * 16 + rl;
+        if (row < nrows && l < L) {
+          int b, r;
+          row_field(m, row, &b, &r);
+          modal[((int64_t)b * Rrows + r) * L + l] = os[rl * kOutPitch + col];
+        }
       }
     }
   }
@@ -332,18 +353,19 @@ __global__ void forward_fragments_kernel(int L, int Jh, int Jhp, int ks,
 
 template <bool BF, int KS>
 int launch_rows(const dino_plan* pl, const float* in, float* out, int64_t B, hipStream_t s) {
-  const bool aligned = (reinterpret_cast<uintptr_t>(in) % 16 == 0) &&
-                       (reinterpret_cast<uintptr_t>(out) % 16 == 0);
-  const bool vec = pl->J % 8 == 0 && pl->L % 4 == 0 && aligned;
+  const bool vin = pl->J % 8 == 0 && reinterpret_cast<uintptr_t>(in) % 16 == 0;
+  const bool vout = pl->L % 4 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0;
   const dim3 grid((unsigned)((2 * B + kRowsPerBlock - 1) / kRowsPerBlock), (unsigned)pl->M);
   const float4* tab = BF ? reinterpret_cast<const float4*>(pl->ffragbf)
                          : reinterpret_cast<const float4*>(pl->ffrag32);
-  if (vec)
-    leg_fwd_rows_kernel<BF, KS, true><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, (int)B,
-                                                            pl->L, pl->J, pl->Jh, pl->R);
-  else
-    leg_fwd_rows_kernel<BF, KS, false><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, (int)B,
-                                                             pl->L, pl->J, pl->Jh, pl->R);
+#define ROWS_LAUNCH(VI, VO)                                                                  \
+  leg_fwd_rows_kernel<BF, KS, VI, VO><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, (int)B, \
+                                                            pl->L, pl->J, pl->Jh, pl->R)
+  if (vin && vout) ROWS_LAUNCH(true, true);
+  else if (vin) ROWS_LAUNCH(true, false);
+  else if (vout) ROWS_LAUNCH(false, true);
+  else ROWS_LAUNCH(false, false);
+#undef ROWS_LAUNCH
   DINO_LAUNCH_CHECK("legendre forward rows kernel");
   return DINO_OK;
 }

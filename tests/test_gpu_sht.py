@@ -28,6 +28,10 @@ def _grids():
       'TL127': (sh.Grid.TL127(), osht.Grid.TL127()),
       'equiangular': (sh.Grid(24, 24, 96, 48, latitude_spacing='equiangular'),
                       osht.Grid(24, 24, 96, 48, latitude_spacing='equiangular')),
+      # odd J (equator node), and J % 8 != 0 with L % 4 == 0: the scalar-load
+      # and vector-store paths of the FP32 / split Legendre kernels
+      'odd_j25': (sh.Grid(12, 13, 32, 25), osht.Grid(12, 13, 32, 25)),
+      'j36_l16': (sh.Grid(16, 16, 48, 36), osht.Grid(16, 16, 48, 36)),
   }
 
 
@@ -54,7 +58,8 @@ def test_transforms_fp64_match_oracle(name, gpu_device):
   assert _rel(z[..., :-1], x[..., :-1]) < 1e-11
 
 
-@pytest.mark.parametrize('name', ['T21', 'wn16_direct', 'TL127'])
+@pytest.mark.parametrize('name', ['T21', 'wn16_direct', 'TL127', 'equiangular', 'odd_j25',
+                                  'j36_l16'])
 def test_transforms_fp32(name, gpu_device):
   g, og = _grids()[name]
   print(f'[test] fp32 transforms {name}', flush=True)
@@ -134,14 +139,21 @@ def test_grid_operators_match_oracle(gpu_device):
   assert _rel(d, og.div_cos_lat((x[0], x[1]))) < 1e-14
 
 
-@pytest.mark.parametrize('name', ['T21', 'TL127'])
+@pytest.mark.parametrize('name', ['T21', 'TL127', 'odd_j25', 'j36_l16', 'equiangular'])
 def test_split_bf16x3_transforms_within_1e5(name, gpu_device):
   """Precision pattern: split-BF16 x3 MFMA Legendre with FP32 accumulation
   stays within 1e-5 normwise (max-abs and RMS) of the FP64 oracle."""
   import functools
   impl = functools.partial(sh.RealSphericalHarmonics, transform_precision='bf16x3')
-  g = getattr(sh.Grid, name)(spherical_harmonics_impl=impl)
-  og = getattr(osht.Grid, name)()
+  shapes = {'odd_j25': ((12, 13, 32, 25), {}), 'j36_l16': ((16, 16, 48, 36), {}),
+            'equiangular': ((24, 24, 96, 48), {'latitude_spacing': 'equiangular'})}
+  if name in shapes:
+    args, kw = shapes[name]
+    g = sh.Grid(*args, spherical_harmonics_impl=impl, **kw)
+    og = osht.Grid(*args, **kw)
+  else:
+    g = getattr(sh.Grid, name)(spherical_harmonics_impl=impl)
+    og = getattr(osht.Grid, name)()
   x = _modal(og, 6, 9)
   for i, s in enumerate([1.0, 1e-2, 300.0, 1e-5, 1.0, 1.0]):
     x[i] *= s

@@ -68,6 +68,12 @@ struct dino_plan {
   int64_t* ftile_off;   // (M) first tile of each m, in 16-byte units
   float* ffrag32;       // FP32 fragments
   uint16_t* ffragbf;    // BF16 hi/lo fragments
+  // Inverse-transform tables in MFMA fragment order (legendre_inv_rows.hip).
+  // int_nt = 16-node tiles the tables hold (1, 2, 4 or 8; 0 when Jh > 128).
+  int int_nt;
+  int64_t* itile_off;   // (M) first tile of each m, in 16-byte units
+  float* ifrag32;       // FP32 fragments
+  float* ifragbf;       // BF16 hi/lo fragments (16-byte slots)
 };
 
 struct dino_implicit {

@@ -358,13 +358,17 @@ int inv_by_width(const dino_plan* pl, const float* in, float* out, int64_t B, hi
 
 int legendre_forward_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
                           bool split, hipStream_t s);
+int legendre_inverse_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
+                          bool split, hipStream_t s);
 
 int legendre_inverse_mfma(const dino_plan* pl, const void* modal, void* fourier, int64_t B,
                           int precision, hipStream_t s) {
   if (B == 0) return DINO_OK;
-  DINO_REQUIRE((2 * B + TR - 1) / TR <= 65535, DINO_EINVAL, "batch too large");
   const float* in = static_cast<const float*>(modal);
   float* out = static_cast<float*>(fourier);
+  if (pl->int_nt > 0 && (precision == DINO_PREC_NATIVE || precision == DINO_PREC_BF16X3))
+    return legendre_inverse_rows(pl, in, out, B, precision == DINO_PREC_BF16X3, s);
+  DINO_REQUIRE((2 * B + TR - 1) / TR <= 65535, DINO_EINVAL, "batch too large");
   switch (precision) {
     case DINO_PREC_NATIVE: return inv_by_width<PolicyF32, 16>(pl, in, out, B, s);
     case DINO_PREC_BF16X3: return inv_by_width<PolicySplit<true>, 32>(pl, in, out, B, s);

@@ -175,12 +175,16 @@ __global__ void quad_weights_kernel(int I, int J, const double* w, double* q64,
 int forward_fragment_steps(int Jh);
 int64_t forward_fragment_layout(int M, int L, int ks, int64_t* off);
 void forward_fragment_fill(const dino_plan* pl, hipStream_t s);
+int inverse_fragment_tiles(int Jh);
+int64_t inverse_fragment_layout(int M, int L, int nt, int64_t* off);
+void inverse_fragment_fill(const dino_plan* pl, hipStream_t s);
 
 static void free_plan(dino_plan* p) {
   if (!p) return;
   void* bufs[] = {p->nodes64, p->weights64, p->qw32, p->qw64, p->m_offset,
                   p->p64, p->p32, p->p16_hi, p->p16_lo, p->pbf_hi, p->pbf_lo,
-                  p->tw64, p->tw32, p->ftile_off, p->ffrag32, p->ffragbf};
+                  p->tw64, p->tw32, p->ftile_off, p->ffrag32, p->ffragbf,
+                  p->itile_off, p->ifrag32, p->ifragbf};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete p;
@@ -295,6 +299,17 @@ extern "C" int dino_plan_create(int M, int L, int I, int J, const double* nodes,
     PLAN_HIP(hipMalloc(&p->ffragbf, 16 * f4));
     PLAN_HIP(hipMemcpy(p->ftile_off, foff.data(), sizeof(int64_t) * M, hipMemcpyHostToDevice));
     forward_fragment_fill(p, 0);
+    PLAN_HIP(hipGetLastError());
+  }
+  p->int_nt = inverse_fragment_tiles(p->Jh);
+  if (p->int_nt > 0) {
+    std::vector<int64_t> ioff(M);
+    const int64_t f4 = inverse_fragment_layout(M, L, p->int_nt, ioff.data());
+    PLAN_HIP(hipMalloc(&p->itile_off, sizeof(int64_t) * M));
+    PLAN_HIP(hipMalloc(&p->ifrag32, 16 * f4));
+    PLAN_HIP(hipMalloc(&p->ifragbf, 16 * f4));
+    PLAN_HIP(hipMemcpy(p->itile_off, ioff.data(), sizeof(int64_t) * M, hipMemcpyHostToDevice));
+    inverse_fragment_fill(p, 0);
     PLAN_HIP(hipGetLastError());
   }
   PLAN_HIP(hipDeviceSynchronize());

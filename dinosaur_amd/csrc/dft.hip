@@ -12,6 +12,7 @@
 //
 // Real-basis conventions (fourier.py:24-60): row 0 is 1/sqrt(2 pi), row 2m-1
 // is cos(m lambda)/sqrt(pi), row 2m is sin(m lambda)/sqrt(pi).
+#include <algorithm>
 #include <type_traits>
 
 #include "dino_common.h"
@@ -321,6 +322,154 @@ __global__ __launch_bounds__(NT) void dft_inverse_fft_kernel(
                               weighted ? qw : nullptr);
 }
 
+// ---------------------------------------------------------------- persistent FP32 FFT
+// The workhorse shape (FP32, power-of-two I <= 512, J % 32 == 0, 16-byte
+// aligned buffers): persistent workgroups walk the (field, 32-column) tiles
+// and prefetch the next tile's input into registers while the current tile
+// is transformed in LDS, so HBM reads overlap the FFT passes and the stores
+// instead of alternating with them.  Same packing, passes and unpacking as the
+// kernels above, hence the same results.
+typedef float f4v __attribute__((ext_vector_type(4)));
+constexpr int kFftThreads = 256;
+constexpr int kPersistThreads = 512;  // halves the per-thread prefetch and pass registers
+
+template <int N>
+__global__ __launch_bounds__(kPersistThreads) void fft_forward_persistent_kernel(
+    const float* __restrict__ nodal, float* __restrict__ fourier, const float* __restrict__ tw,
+    const float* __restrict__ qw, int J, int Rrows, int weighted, int64_t ntiles, int tpf) {
+  constexpr int P = 16, NT = kPersistThreads, CV = 8, ROWS = NT / CV, LPT = N / ROWS;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<float>* buf = reinterpret_cast<cplx<float>*>(smem_raw);
+  cplx<float>* tws = buf + N * P;
+  const int tid = threadIdx.x, cv = tid % CV, n0 = tid / CV;
+  for (int k = tid; k < N; k += NT) tws[k] = cplx<float>{tw[2 * k], -tw[2 * k + 1]};
+  int64_t tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  f4v pre[LPT];
+  auto issue = [&](int64_t t) {
+    const int64_t b = t / tpf;
+    const int j0 = (int)(t - b * tpf) * 32;
+    const float* src = nodal + b * (int64_t)N * J + j0 + 4 * cv;
+#pragma unroll
+    for (int q = 0; q < LPT; ++q)
+      pre[q] = *reinterpret_cast<const f4v*>(src + (int64_t)(n0 + ROWS * q) * J);
+  };
+  issue(tile);
+  const float inv_sqrt_2pi = 0.39894228040143267794f;
+  const float inv_sqrt_pi = 0.56418958354775628695f;
+  for (; tile < ntiles; tile += gridDim.x) {
+    const int64_t b = tile / tpf;
+    const int j0 = (int)(tile - b * tpf) * 32;
+    f4v w = {1.f, 1.f, 1.f, 1.f};
+    if (weighted) w = *reinterpret_cast<const f4v*>(qw + j0 + 4 * cv);
+    // column pair (2p, 2p + 1) -> complex (x_a, x_b): a float4 is two pairs
+#pragma unroll
+    for (int q = 0; q < LPT; ++q)
+      *reinterpret_cast<f4v*>(&buf[(n0 + ROWS * q) * P + 2 * cv]) = pre[q] * w;
+    __syncthreads();
+    if (tile + gridDim.x < ntiles) issue(tile + gridDim.x);
+    fft_lds<float, N, P, NT, false>(buf, tws);
+    float* dst = fourier + b * (int64_t)Rrows * J + j0;
+    for (int idx = tid; idx < Rrows * CV; idx += NT) {
+      const int r = idx / CV, c4 = idx % CV;
+      const int m = (r + 1) / 2;
+      f4v o;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int c = 4 * c4 + e;
+        const cplx<float> zm = buf[m * P + c / 2];
+        const cplx<float> zn = buf[((N - m) % N) * P + c / 2];
+        float re, im;
+        if ((c & 1) == 0) {
+          re = (zm.x + zn.x) * 0.5f;
+          im = (zm.y - zn.y) * 0.5f;
+        } else {
+          re = (zm.y + zn.y) * 0.5f;
+          im = -(zm.x - zn.x) * 0.5f;
+        }
+        o[e] = r == 0 ? re * inv_sqrt_2pi : ((r & 1) ? re * inv_sqrt_pi : -im * inv_sqrt_pi);
+      }
+      *reinterpret_cast<f4v*>(dst + (int64_t)r * J + 4 * c4) = o;
+    }
+    __syncthreads();  // buf is rewritten by the next tile
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(kPersistThreads) void fft_inverse_persistent_kernel(
+    const float* __restrict__ fourier, float* __restrict__ nodal, const float* __restrict__ tw,
+    const float* __restrict__ qw, int J, int M, int weighted, int64_t ntiles, int tpf) {
+  constexpr int P = 16, NT = kPersistThreads, CV = 8, ROWS = NT / CV, LPT = N / ROWS;
+  constexpr int MAXM = N / 2;                     // M <= N / 2 + 1 rows of input are packed
+  constexpr int IPT = (MAXM * CV + NT - 1) / NT;  // (m, column quad) items per thread
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<float>* buf = reinterpret_cast<cplx<float>*>(smem_raw);
+  cplx<float>* tws = buf + N * P;
+  const int tid = threadIdx.x;
+  const int Rrows = 2 * M - 1;
+  for (int k = tid; k < N; k += NT) tws[k] = cplx<float>{tw[2 * k], -tw[2 * k + 1]};
+  int64_t tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  f4v g1[IPT], g2[IPT];
+  auto issue = [&](int64_t t) {
+    const int64_t b = t / tpf;
+    const int j0 = (int)(t - b * tpf) * 32;
+    const float* src = fourier + b * (int64_t)Rrows * J + j0;
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      const int idx = tid + NT * i;
+      const int m = idx / CV, c4 = idx % CV;
+      const bool ok = m < M;
+      const int r1 = ok ? (m == 0 ? 0 : 2 * m - 1) : 0;
+      const int r2 = ok ? (m == 0 ? 0 : 2 * m) : 0;
+      g1[i] = *reinterpret_cast<const f4v*>(src + (int64_t)r1 * J + 4 * c4);
+      g2[i] = *reinterpret_cast<const f4v*>(src + (int64_t)r2 * J + 4 * c4);
+    }
+  };
+  issue(tile);
+  const float inv_sqrt_2pi = 0.39894228040143267794f;
+  const float half_inv_sqrt_pi = 0.28209479177387814347f;
+  for (; tile < ntiles; tile += gridDim.x) {
+    const int64_t b = tile / tpf;
+    const int j0 = (int)(tile - b * tpf) * 32;
+    // rows M .. N - M of the spectrum are zero
+    for (int idx = tid; idx < (N - 2 * M + 1) * P; idx += NT)
+      buf[(M + idx / P) * P + idx % P] = cplx<float>{0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      const int idx = tid + NT * i;
+      const int m = idx / CV, c4 = idx % CV;
+      if (m >= M) continue;
+      const f4v a = g1[i], c = g2[i];
+      if (m == 0) {
+        *reinterpret_cast<f4v*>(&buf[2 * c4]) = a * inv_sqrt_2pi;
+      } else {
+        // pairs p = 2 c4 (a.x, a.y | c.x, c.y) and 2 c4 + 1 (a.z, a.w | c.z, c.w)
+        const f4v lo = {(a[0] + c[1]) * half_inv_sqrt_pi, (a[1] - c[0]) * half_inv_sqrt_pi,
+                        (a[2] + c[3]) * half_inv_sqrt_pi, (a[3] - c[2]) * half_inv_sqrt_pi};
+        const f4v hi = {(a[0] - c[1]) * half_inv_sqrt_pi, (c[0] + a[1]) * half_inv_sqrt_pi,
+                        (a[2] - c[3]) * half_inv_sqrt_pi, (c[2] + a[3]) * half_inv_sqrt_pi};
+        *reinterpret_cast<f4v*>(&buf[m * P + 2 * c4]) = lo;
+        *reinterpret_cast<f4v*>(&buf[(N - m) * P + 2 * c4]) = hi;
+      }
+    }
+    __syncthreads();
+    if (tile + gridDim.x < ntiles) issue(tile + gridDim.x);
+    fft_lds<float, N, P, NT, true>(buf, tws);
+    float* dst = nodal + b * (int64_t)N * J + j0;
+    const int cv = tid % CV, n0 = tid / CV;
+    f4v w = {1.f, 1.f, 1.f, 1.f};
+    if (weighted) w = *reinterpret_cast<const f4v*>(qw + j0 + 4 * cv);
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
+      const int n = n0 + ROWS * q;
+      *reinterpret_cast<f4v*>(dst + (int64_t)n * J + 4 * cv) =
+          *reinterpret_cast<const f4v*>(&buf[n * P + 2 * cv]) * w;
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- direct DFT
 template <typename T>
 __global__ void dft_forward_direct_kernel(const T* __restrict__ nodal, T* __restrict__ fourier,
@@ -401,7 +550,6 @@ __global__ void dft_inverse_direct_kernel(const T* __restrict__ fourier, T* __re
   }
 }
 
-constexpr int kFftThreads = 256;
 
 template <typename T>
 int fft_pairs_for(int N) {
@@ -430,9 +578,53 @@ int launch_fft_fixed(bool forward, const dino_plan* pl, const T* in, T* out,
   return DINO_OK;
 }
 
+int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0)
+      n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
+template <int N>
+int launch_fft_persistent(bool forward, const dino_plan* pl, const float* in, float* out,
+                          const float* tw, const float* qw, int64_t B, int weighted,
+                          hipStream_t s) {
+  const int tpf = pl->J / 32;
+  const int64_t ntiles = B * tpf;
+  const size_t lds = sizeof(cplx<float>) * (size_t)N * 17;
+  // one resident wave of workgroups: as many per CU as registers and LDS allow
+  auto kf = fft_forward_persistent_kernel<N>;
+  auto ki = fft_inverse_persistent_kernel<N>;
+  const void* k = forward ? (const void*)kf : (const void*)ki;
+  if (lds > 65536)
+    DINO_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int per_cu = 0;
+  DINO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kPersistThreads, lds));
+  const int64_t grid = std::min<int64_t>(ntiles, (int64_t)std::max(per_cu, 1) * device_cus());
+  if (forward)
+    kf<<<(unsigned)grid, kPersistThreads, lds, s>>>(in, out, tw, qw, pl->J, pl->R, weighted, ntiles, tpf);
+  else
+    ki<<<(unsigned)grid, kPersistThreads, lds, s>>>(in, out, tw, qw, pl->J, pl->M, weighted, ntiles, tpf);
+  DINO_LAUNCH_CHECK("persistent fft kernel");
+  return DINO_OK;
+}
+
 template <typename T, int N>
 int launch_fft_n(bool forward, const dino_plan* pl, const T* in, T* out, const T* tw,
                  const T* qw, int64_t B, int weighted, hipStream_t s) {
+  if constexpr (sizeof(T) == 4 && N >= 128 && N <= 512) {
+    const bool aligned = reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
+                         reinterpret_cast<uintptr_t>(out) % 16 == 0;
+    if (pl->J % 32 == 0 && aligned && 2 * pl->M - 1 <= N)
+      return launch_fft_persistent<N>(forward, pl, in, out, tw, qw, B, weighted, s);
+  }
   int P = fft_pairs_for<T>(N);
   switch (P) {
     case 16: return launch_fft_fixed<T, N, 16>(forward, pl, in, out, tw, qw, B, weighted, s);

@@ -291,10 +291,16 @@ __global__ __launch_bounds__(256) void nodal_forward_kernel(
 // cotangents of those totals (sum ss_k Wbar_k, sum ss_k Xbar_k, -lsp bar) and
 // the moist geopotential cotangent total; pass 2 (up the column) carries the
 // suffix sums that are the adjoints of the downward prefix sums.
-template <typename T>
+// Levels are streamed through a register window: pass 1 keeps levels
+// (k, k + 1) for the interface terms, pass 2 (upwards) keeps k + 1, k and a
+// prefetched k - 1, so every value is loaded once per pass.  NTR is the
+// tracer count (the humidity and cloud indices select among them with
+// unrolled compares, never a runtime register index).
+template <typename T, int NTR>
 __global__ __launch_bounds__(256) void nodal_adjoint_kernel(
     const T* __restrict__ in, const T* __restrict__ gout, T* __restrict__ gin,
     const T* __restrict__ cst, Groups g, Params p) {
+  constexpr int NA = NTR > 0 ? NTR : 1;
   const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t e = blockIdx.y;
   if (col >= p.IJ) return;
@@ -317,52 +323,144 @@ __global__ __launch_bounds__(256) void nodal_adjoint_kernel(
   const T R = (T)p.R, kappa = (T)p.kappa;
   const bool moist = p.moist != 0, iva = p.iva != 0;
   const T lp = moist ? X_(g.lp, 1, 0) : (T)0;
-  // cotangent of vadv(W, x) at level k for each advected x
-  auto vb_u = [&](int k) -> T { return iva ? -s * B_(g.cu, K, k) : (T)0; };
-  auto vb_v = [&](int k) -> T { return iva ? -s * B_(g.cv, K, k) : (T)0; };
-  auto vb_t = [&](int k) -> T { return iva ? B_(g.te, K, k) : (T)0; };
-  auto vb_tr = [&](int i, int k) -> T { return iva ? B_(g.trn[i], K, k) : (T)0; };
-  // Wbar at interface jj (0 <= jj <= K-2) = sum_x Fbar_x (x_{jj+1} - x_jj) c_jj
-  auto wbar = [&](int jj) -> T {
-    T c = cst[o.c2c + jj];
+  const int hum = p.hum;
+
+  // per-level values: "light" = what the interface terms of a neighbour need
+  struct XL {
+    T u, v, t, tr[NA];
+  };
+  struct BL {
+    T cu, cv, te, trn[NA];
+  };
+  struct XF {
+    XL l;
+    T d, z, qx, qy;
+  };
+  struct BF {
+    BL l;
+    T ke, ut, vt, tu[NA], tv[NA], curl, dc, geo;
+  };
+  auto load_xl = [&](int k, XL& x) {
+    x.u = X_(g.u, K, k);
+    x.v = X_(g.v, K, k);
+    x.t = X_(g.t, K, k);
+#pragma unroll
+    for (int i = 0; i < NTR; ++i) x.tr[i] = X_(g.tr[i], K, k);
+  };
+  auto load_bl = [&](int k, BL& b) {
+    b.cu = B_(g.cu, K, k);
+    b.cv = B_(g.cv, K, k);
+    b.te = B_(g.te, K, k);
+#pragma unroll
+    for (int i = 0; i < NTR; ++i) b.trn[i] = B_(g.trn[i], K, k);
+  };
+  auto load_xf = [&](int k, XF& x) {
+    load_xl(k, x.l);
+    x.d = X_(g.d, K, k);
+    x.z = X_(g.z, K, k);
+    x.qx = moist ? X_(g.qx, K, k) : (T)0;
+    x.qy = moist ? X_(g.qy, K, k) : (T)0;
+  };
+  auto load_bf = [&](int k, BF& b) {
+    load_bl(k, b.l);
+    b.ke = B_(g.ke, K, k);
+    b.ut = B_(g.ut, K, k);
+    b.vt = B_(g.vt, K, k);
+#pragma unroll
+    for (int i = 0; i < NTR; ++i) {
+      b.tu[i] = B_(g.tu[i], K, k);
+      b.tv[i] = B_(g.tv[i], K, k);
+    }
+    b.curl = moist ? B_(g.curl, K, k) : (T)0;
+    b.dc = moist ? B_(g.dc, K, k) : (T)0;
+    b.geo = moist ? B_(g.geo, K, k) : (T)0;
+  };
+  // cotangents of vadv(W, x) at a level
+  auto vb_u = [&](const BL& b) -> T { return iva ? -s * b.cu : (T)0; };
+  auto vb_v = [&](const BL& b) -> T { return iva ? -s * b.cv : (T)0; };
+  auto vb_t = [&](const BL& b) -> T { return iva ? b.te : (T)0; };
+  auto vb_tr = [&](const BL& b, int i) -> T { return iva ? b.trn[i] : (T)0; };
+  // Wbar at interface k = sum_x Fbar_x (x_{k+1} - x_k) c_k  (levels k, k + 1)
+  auto wbar = [&](int k, const XL& x0, const XL& x1, const BL& b0, const BL& b1) -> T {
+    T c = cst[o.c2c + k];
     T acc = 0;
-    acc += (T)-0.5 * (vb_u(jj) + vb_u(jj + 1)) * (X_(g.u, K, jj + 1) - X_(g.u, K, jj));
-    acc += (T)-0.5 * (vb_v(jj) + vb_v(jj + 1)) * (X_(g.v, K, jj + 1) - X_(g.v, K, jj));
-    acc += (T)-0.5 * (vb_t(jj) + vb_t(jj + 1)) * (X_(g.t, K, jj + 1) - X_(g.t, K, jj));
-    for (int i = 0; i < p.nt; ++i)
-      acc += (T)-0.5 * (vb_tr(i, jj) + vb_tr(i, jj + 1)) *
-             (X_(g.tr[i], K, jj + 1) - X_(g.tr[i], K, jj));
+    acc += (T)-0.5 * (vb_u(b0) + vb_u(b1)) * (x1.u - x0.u);
+    acc += (T)-0.5 * (vb_v(b0) + vb_v(b1)) * (x1.v - x0.v);
+    acc += (T)-0.5 * (vb_t(b0) + vb_t(b1)) * (x1.t - x0.t);
+#pragma unroll
+    for (int i = 0; i < NTR; ++i)
+      acc += (T)-0.5 * (vb_tr(b0, i) + vb_tr(b1, i)) * (x1.tr[i] - x0.tr[i]);
     return acc * c;
   };
-  auto xbar = [&](int jj) -> T {
+  auto xbar = [&](int k, const BL& b0, const BL& b1) -> T {
     if (!p.tnc) return (T)0;
-    T c = cst[o.c2c + jj];
-    return (T)-0.5 * (B_(g.te, K, jj) + B_(g.te, K, jj + 1)) *
-           (cst[o.tref + jj + 1] - cst[o.tref + jj]) * c;
+    T c = cst[o.c2c + k];
+    return (T)-0.5 * (b0.te + b1.te) * (cst[o.tref + k + 1] - cst[o.tref + k]) * c;
   };
-  // ---- pass 1
+  // humidity and the cloud sum of a level (unrolled selects)
+  auto hum_of = [&](const XL& x) -> T {
+    T q = 0;
+#pragma unroll
+    for (int i = 0; i < NTR; ++i) q = (i == hum) ? x.tr[i] : q;
+    return q;
+  };
+  auto is_cloud = [&](int i) -> bool {
+    bool r = false;
+    for (int c2 = 0; c2 < p.ncloud; ++c2) r = r || (p.cloud[c2] == i);
+    return r;
+  };
+  auto tv_of = [&](const XL& x, int k) -> T {
+    if (!moist) return x.t;
+    T qq = hum_of(x);
+    T dn = (T)1 + (T)(p.hcr - 1) * qq;
+    return x.t * (((T)1 + (T)(p.gcr - 1) * qq) / dn) +
+           cst[o.tref + k] * (((T)(p.gcr - p.hcr) * qq) / dn);
+  };
+
+  // ---- pass 1: totals and the cotangents of the totals
   T TE = 0, TF = 0, TFb = 0, TEb = -B_(g.lsp, 1, 0), GEOT = 0;
-  for (int k = 0; k < K; ++k) {
-    T u = X_(g.u, K, k), v = X_(g.v, K, k), d = X_(g.d, K, k);
-    T A = (u * gx + v * gy) * s;
-    T ds = cst[o.dsig + k];
-    TE += A * ds;
-    TF += (d + A) * ds;
-    if (k < K - 1) {
-      TFb += cst[o.ss + k] * wbar(k);
-      TEb += cst[o.ss + k] * xbar(k);
+  {
+    XL xa, xb;
+    BL ba, bb;
+    load_xl(0, xa);
+    load_bl(0, ba);
+    for (int k = 0; k < K; ++k) {
+      const int kn = k + 1 < K ? k + 1 : k;
+      load_xl(kn, xb);
+      load_bl(kn, bb);
+      const T d = X_(g.d, K, k);
+      const T A = (xa.u * gx + xa.v * gy) * s;
+      const T ds = cst[o.dsig + k];
+      TE += A * ds;
+      TF += (d + A) * ds;
+      if (k < K - 1) {
+        TFb += cst[o.ss + k] * wbar(k, xa, xb, ba, bb);
+        TEb += cst[o.ss + k] * xbar(k, ba, bb);
+      }
+      if (moist) GEOT += B_(g.geo, K, k);
+      xa = xb;
+      ba = bb;
     }
-    if (moist) GEOT += B_(g.geo, K, k);
   }
   // ---- pass 2: k = K-1 .. 0
   T gxb = 0, gyb = 0, lpb = 0;
-  T SPF = 0, SPE = 0;        // suffix sums of PFbar, PEbar over k' >= k
-  T SB = 0, SE = 0;          // suffix sums of b, e over k' > k (forward values)
-  T SGEO = 0;                // suffix sum of geo bars over k' >= k
+  T SPF = 0, SPE = 0;  // suffix sums of PFbar, PEbar over k' >= k
+  T SB = 0, SE = 0;    // suffix sums of b, e over k' > k (forward values)
+  T SGEO = 0;          // suffix sum of geo bars over k' >= k
+  XF xc, xn;
+  BF bc, bn;
+  XL xu = {};
+  BL bu = {};
+  T tv_up = 0;
+  load_xf(K - 1, xc);
+  load_bf(K - 1, bc);
   for (int k = K - 1; k >= 0; --k) {
+    const int kd = k > 0 ? k - 1 : k;  // level k - 1, prefetched for the next step
+    load_xf(kd, xn);
+    load_bf(kd, bn);
     const T ds = cst[o.dsig + k];
-    const T u = X_(g.u, K, k), v = X_(g.v, K, k), d = X_(g.d, K, k), t = X_(g.t, K, k);
-    const T z = X_(g.z, K, k);
+    const T u = xc.l.u, v = xc.l.v, d = xc.d, t = xc.l.t;
+    const T z = xc.z;
     const T A = (u * gx + v * gy) * s;
     const T b = (d + A) * ds, ev = A * ds;
     const T PF = TF - SB, PE = TE - SE;    // prefixes at k
@@ -376,34 +474,28 @@ __global__ __launch_bounds__(256) void nodal_adjoint_kernel(
     // humidity quantities at k
     T q = 0, adj = 1, den = 1;
     if (moist) {
-      q = X_(g.tr[p.hum], K, k);
+      q = hum_of(xc.l);
       adj = (T)1 + (T)(p.gcr - 1) * q;
-      for (int c2 = 0; c2 < p.ncloud; ++c2) adj -= X_(g.tr[p.cloud[c2]], K, k);
+#pragma unroll
+      for (int i = 0; i < NTR; ++i)
+        if (is_cloud(i)) adj -= xc.l.tr[i];
       den = (T)1 + (T)(p.hcr - 1) * q;
     }
-    auto tv_at = [&](int kk) -> T {
-      T tt = X_(g.t, K, kk);
-      if (!moist) return tt;
-      T qq = X_(g.tr[p.hum], K, kk);
-      T dn = (T)1 + (T)(p.hcr - 1) * qq;
-      return tt * (((T)1 + (T)(p.gcr - 1) * qq) / dn) +
-             cst[o.tref + kk] * (((T)(p.gcr - p.hcr) * qq) / dn);
-    };
-    const T tv = tv_at(k);
+    const T tv = tv_of(xc.l, k);
     // output cotangents at k
-    const T cu = B_(g.cu, K, k), cv = B_(g.cv, K, k), ke = B_(g.ke, K, k);
-    const T te = B_(g.te, K, k), ut = B_(g.ut, K, k), vt = B_(g.vt, K, k);
+    const T cu = bc.l.cu, cv = bc.l.cv, ke = bc.ke;
+    const T te = bc.l.te, ut = bc.ut, vt = bc.vt;
     const T om = kappa * te;  // cotangent of both T omega / p terms
     // prefix cotangents from T omega / p at levels k and k+1
     T pfb = -al * tv * om / ds;
     T peb = -al * tref * om / ds;
     if (k < K - 1) {
       T ds1 = cst[o.dsig + k + 1];
-      T om1 = kappa * B_(g.te, K, k + 1);
-      pfb += -al * tv_at(k + 1) * om1 / ds1;
+      T om1 = kappa * bu.te;
+      pfb += -al * tv_up * om1 / ds1;
       peb += -al * cst[o.tref + k + 1] * om1 / ds1;
-      pfb += -wbar(k);
-      peb += -xbar(k);
+      pfb += -wbar(k, xc.l, xu, bc.l, bu);
+      peb += -xbar(k, bc.l, bu);
     }
     SPF += pfb;
     SPE += peb;
@@ -457,34 +549,31 @@ __global__ __launch_bounds__(256) void nodal_adjoint_kernel(
       if (k > 0) r += (T)-0.5 * (vbkm + vbk) * (Wm * cm);
       return r;
     };
-    {
-      T a0 = vb_u(k), a1 = k < K - 1 ? vb_u(k + 1) : (T)0, am = k > 0 ? vb_u(k - 1) : (T)0;
-      ub += vadv_bar(a0, a1, am);
-      a0 = vb_v(k), a1 = k < K - 1 ? vb_v(k + 1) : (T)0, am = k > 0 ? vb_v(k - 1) : (T)0;
-      vb += vadv_bar(a0, a1, am);
-      a0 = vb_t(k), a1 = k < K - 1 ? vb_t(k + 1) : (T)0, am = k > 0 ? vb_t(k - 1) : (T)0;
-      tb += vadv_bar(a0, a1, am);
-    }
+    const bool hu = k < K - 1, hd = k > 0;
+    ub += vadv_bar(vb_u(bc.l), hu ? vb_u(bu) : (T)0, hd ? vb_u(bn.l) : (T)0);
+    vb += vadv_bar(vb_v(bc.l), hu ? vb_v(bu) : (T)0, hd ? vb_v(bn.l) : (T)0);
+    tb += vadv_bar(vb_t(bc.l), hu ? vb_t(bu) : (T)0, hd ? vb_t(bn.l) : (T)0);
     // tracers
-    for (int i = 0; i < MAXTR; ++i) {
-      if (i >= p.nt) break;
-      T x = X_(g.tr[i], K, k);
-      T trn = B_(g.trn[i], K, k), tu = B_(g.tu[i], K, k), tvv = B_(g.tv[i], K, k);
+    T trb[NA];
+#pragma unroll
+    for (int i = 0; i < NTR; ++i) {
+      T x = xc.l.tr[i];
+      T trn = bc.l.trn[i], tu = bc.tu[i], tvv = bc.tv[i];
       T xb = d * trn + u * s * tu + v * s * tvv;
       db += x * trn;
       ub += x * s * tu;
       vb += x * s * tvv;
-      T a0 = vb_tr(i, k), a1 = k < K - 1 ? vb_tr(i, k + 1) : (T)0;
-      T am = k > 0 ? vb_tr(i, k - 1) : (T)0;
-      xb += vadv_bar(a0, a1, am);
-      G_(g.tr[i], K, k) = xb;
+      xb += vadv_bar(vb_tr(bc.l, i), hu ? vb_tr(bu, i) : (T)0, hd ? vb_tr(bn.l, i) : (T)0);
+      trb[i] = xb;
     }
     if (moist) {
       // virtual-temperature adjustment of R T
       qb += (T)(p.gcr - 1) * adjb;
-      for (int c2 = 0; c2 < p.ncloud; ++c2) G_(g.tr[p.cloud[c2]], K, k) -= adjb;
-      T qx = X_(g.qx, K, k), qy = X_(g.qy, K, k);
-      T curl = B_(g.curl, K, k), dcb = B_(g.dc, K, k), geob = B_(g.geo, K, k);
+#pragma unroll
+      for (int i = 0; i < NTR; ++i)
+        if (is_cloud(i)) trb[i] -= adjb;
+      T qx = xc.qx, qy = xc.qy;
+      T curl = bc.curl, dcb = bc.dc, geob = bc.geo;
       T cf = tref * (T)p.rvr;
       gxb += cf * s * (qy * curl + qx * dcb);
       gyb += cf * s * (-qx * curl + qy * dcb);
@@ -497,8 +586,12 @@ __global__ __launch_bounds__(256) void nodal_adjoint_kernel(
       T tdb = cst[o.gd + k] * geob + (k >= 1 ? cst[o.gp + k] * (GEOT - SGEO) : (T)0);
       qb += (t + tref) * (T)p.gcr1 * tdb;
       tb += q * (T)p.gcr1 * tdb;
-      G_(g.tr[p.hum], K, k) += qb;
+#pragma unroll
+      for (int i = 0; i < NTR; ++i)
+        if (i == hum) trb[i] += qb;
     }
+#pragma unroll
+    for (int i = 0; i < NTR; ++i) G_(g.tr[i], K, k) = trb[i];
     G_(g.z, K, k) = zb;
     G_(g.d, K, k) = db;
     G_(g.t, K, k) = tb;
@@ -506,6 +599,12 @@ __global__ __launch_bounds__(256) void nodal_adjoint_kernel(
     G_(g.v, K, k) = vb;
     SB += b;
     SE += ev;
+    // slide the window down one level
+    xu = xc.l;
+    bu = bc.l;
+    tv_up = tv;
+    xc = xn;
+    bc = bn;
   }
   G_(g.gx, 1, 0) = gxb;
   G_(g.gy, 1, 0) = gyb;
@@ -543,9 +642,19 @@ int launch(bool adjoint, const dino_nodal* n, const void* in, const void* gout, 
     nodal_forward_kernel<T><<<grid, 256, 0, s>>>(static_cast<const T*>(in), static_cast<T*>(out),
                                                  cst, g, p);
   else
-    nodal_adjoint_kernel<T><<<grid, 256, 0, s>>>(static_cast<const T*>(in),
-                                                 static_cast<const T*>(gout),
-                                                 static_cast<T*>(out), cst, g, p);
+    switch (n->nt) {
+#define ADJ_CASE(NTR)                                                                 \
+  case NTR:                                                                           \
+    nodal_adjoint_kernel<T, NTR><<<grid, 256, 0, s>>>(static_cast<const T*>(in),      \
+                                                      static_cast<const T*>(gout),    \
+                                                      static_cast<T*>(out), cst, g, p); \
+    break;
+      ADJ_CASE(0) ADJ_CASE(1) ADJ_CASE(2) ADJ_CASE(3) ADJ_CASE(4) ADJ_CASE(5) ADJ_CASE(6)
+#undef ADJ_CASE
+      default:
+        set_error("nodal kernel supports up to %d tracers", MAXTR);
+        return DINO_EINVAL;
+    }
   DINO_LAUNCH_CHECK("nodal kernel");
   return DINO_OK;
 }

@@ -429,6 +429,19 @@ def main():
   roof['kernel'] = dom
   roof['algorithmic_bytes'] = work
 
+  # forward-only rollout replayed from a HIP graph (inference path, same state)
+  from dinosaur_amd import time_integration as ti_g
+  from dinosaur_amd import primitive_equations as pe_g
+  graphed = ti_g.GraphedStep(step, pe_g.to_flat(state))
+  y = pe_g.to_flat(state)
+  _sync(torch)
+  g0 = time.perf_counter()
+  for _ in range(4):
+    y = graphed(y)
+  _sync(torch)
+  graphed_ms = (time.perf_counter() - g0) / 4 * 1e3
+  print(f'[bench] graphed forward step: {graphed_ms:.1f} ms', file=sys.stderr, flush=True)
+  del graphed, y
   plain = None
   del eq, state, step, targets
   torch.cuda.empty_cache()
@@ -478,6 +491,7 @@ def main():
                  'plain_fp32': plain,
                  'chosen_over_plain': (value / plain['value']) if plain else None,
                  'forward_only': args.forward_only,
+                 'graphed_forward_ms_per_step': graphed_ms,
                  'accuracy_vs_fp64': acc},
       'roofline': roof,
       'cpu_baseline': cpu,

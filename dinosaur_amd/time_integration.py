@@ -301,6 +301,57 @@ def trajectory_from_step(step_fn, outer_steps: int, inner_steps: int, *,
   return multistep
 
 
+class GraphedStep:
+  """`steps` applications of `step_fn` captured into one HIP graph.
+
+  The graph owns a static input buffer and a static output buffer (the flat
+  state layout of `example`).  Calling it copies the state into the input
+  buffer, replays the graph and returns a state whose leaves are views of the
+  output buffer: the next call overwrites them, so keep a `clone` of any
+  output that must survive.  `sim_time` advances on the host by the increment
+  observed at capture.  Lazily created objects (transform plans, implicit
+  factors, fragment tables) are built by the warm-up calls, before capture,
+  so nothing allocates or copies from the host inside the graph.  The steps
+  run without autograd (a forward rollout / inference path).
+  """
+
+  def __init__(self, step_fn, example, steps: int = 1, warmup: int = 2):
+    flat = example.as_flat()
+    self._layout = flat._flat[1:]
+    self._cls = type(flat)
+    self._in = flat.flat_buffer().detach().clone()
+    t0 = flat.sim_time
+
+    def run():
+      y = self._cls.from_flat(self._in, *self._layout, sim_time=t0)
+      for _ in range(steps):
+        y = step_fn(y)
+      return y.as_flat()
+
+    with torch.no_grad():
+      side = torch.cuda.Stream()
+      side.wait_stream(torch.cuda.current_stream())
+      with torch.cuda.stream(side):
+        for _ in range(warmup):
+          run()
+      torch.cuda.current_stream().wait_stream(side)
+      self.graph = torch.cuda.CUDAGraph()
+      with torch.cuda.graph(self.graph):
+        y = run()
+    self._out = y.flat_buffer()
+    self._dt = None if (t0 is None or y.sim_time is None) else y.sim_time - t0
+
+  def __call__(self, state):
+    buf = state.as_flat().flat_buffer()
+    if buf.shape != self._in.shape or buf.dtype != self._in.dtype:
+      raise ValueError(f'graphed step expects a state buffer of shape {tuple(self._in.shape)} '
+                       f'and dtype {self._in.dtype}, got {tuple(buf.shape)} {buf.dtype}')
+    self._in.copy_(buf)
+    self.graph.replay()
+    t = None if (state.sim_time is None or self._dt is None) else state.sim_time + self._dt
+    return self._cls.from_flat(self._out, *self._layout, sim_time=t)
+
+
 def _flatten_state(x):
   """State -> (list of tensors, rebuild)."""
   if hasattr(x, '_ARRAYS') and x.flat_buffer() is not None:

@@ -79,8 +79,12 @@ def build_primitive(name: str, layers: int, *, moist: bool = False, members: int
   state_fn, aux = primitive_equations_states.isothermal_rest_atmosphere(
       coords, ps, surface_height=oro, device=device, dtype=dtype)
   base = state_fn()
+  # The orography parameter is prepared in FP64 and rounded once, so runs in
+  # every precision integrate the same seeded model: the rest state's ln ps
+  # balances g*h to round-off, and a parameter carrying FP32 transform error
+  # instead would be an imbalance (a different model), not a pipeline error.
   modal_oro = primitive_equations.truncated_modal_orography(
-      torch.as_tensor(aux['orography'], device=device, dtype=dtype), coords)
+      torch.as_tensor(aux['orography'], device=device, dtype=torch.float64), coords).to(dtype)
   states = []
   for mem in range(members):
     seed = 0 if members == 1 else 100 + mem

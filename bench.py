@@ -57,6 +57,9 @@ def _parse():
                   help='run one stage kernel at the bench shape, then a 256 MiB device '
                        'copy (the calibration kernel for hardware byte counters), and exit')
   ap.add_argument('--reps', type=int, default=10)
+  ap.add_argument('--workload', default='tl255', choices=['tl255', 'ensemble'],
+                  help='tl255: the headline fwd+adjoint line; ensemble: the mixed-truncation '
+                       'ensemble (32 T85 + 24 T42 + 8 TL127, moist L24), forward ticks')
   return ap.parse_args()
 
 
@@ -379,6 +382,61 @@ def cpu_baseline(sample_fields=37):
   }
 
 
+ENSEMBLE = (('T85', 32, 600.0, 1), ('T42', 24, 1200.0, 2), ('TL127', 8, 600.0, 1))
+ENSEMBLE_LAYERS = 24
+
+
+def mixed_ensemble(torch, ticks, warmup):
+  """Member-sim-days/s of the mixed-truncation ensemble advanced tick by tick:
+  eager, graph-captured on one stream, and graph-captured with every group on
+  its own stream (one graph launch per tick).  A tick is 10 simulated minutes
+  for every member (the T42 group takes 20 min steps on even ticks)."""
+  from dinosaur_amd import ensemble
+  from dinosaur_amd import time_integration as ti
+  from dinosaur_amd import units as units_lib
+  from dinosaur_amd import workloads
+  ticks = max(2, ticks - ticks % 2)
+
+  def groups():
+    out, first = [], 0
+    for name, members, dt_s, every in ENSEMBLE:
+      eq, st, coords, ps = workloads.build_primitive(name, ENSEMBLE_LAYERS, moist=True,
+                                                     members=members, first_member=first)
+      dt = ps.nondimensionalize(dt_s * units_lib.scales.units.s)
+      step = ti.step_with_filters(ti.imex_rk_sil3(eq, dt),
+                                  [ti.exponential_step_filter(coords.horizontal, dt)])
+      out.append(ensemble.Group(name, step, st.as_flat(), every))
+      first += members
+    return out
+
+  members = sum(g[1] for g in ENSEMBLE)
+  res = {}
+  for mode, kw in (('eager', dict(graph=False)), ('graph_serial', dict(concurrent=False)),
+                   ('graph_concurrent', dict(concurrent=True))):
+    ens = ensemble.MixedTruncationEnsemble(groups(), **kw)
+    ens.run(2 * max(1, warmup))
+    _sync(torch)
+    t0 = time.perf_counter()
+    ens.run(ticks)
+    _sync(torch)
+    wall = time.perf_counter() - t0
+    res[mode] = {'ms_per_tick': wall / ticks * 1e3,
+                 'member_sim_days_per_s': members * ticks * (600.0 / 86400.0) / wall}
+    print(f'[bench] ensemble {mode}: {res[mode]}', file=sys.stderr, flush=True)
+    del ens
+    torch.cuda.empty_cache()
+  best = res['graph_concurrent']
+  print(json.dumps({
+      'metric': 'mixed-truncation ensemble member-sim-days/s (forward ticks)',
+      'value': best['member_sim_days_per_s'], 'unit': 'member-sim-days/s', 'n_gpus': 1,
+      'steps': ticks, 'warmup': warmup, 'ms_per_step': best['ms_per_tick'],
+      'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+      'data': 'synthetic (seeded rest atmosphere + orography + perturbations + humidity)',
+      'config': {'workload': '32 T85 + 24 T42 + 8 TL127 members, moist L24 (q + 2 clouds), '
+                             'SIL3 10 min (T42 20 min every other tick), one graph launch '
+                             'per tick', 'modes': res}}))
+
+
 def main():
   args = _parse()
   if args.gpus != 1:
@@ -398,6 +456,9 @@ def main():
     precision = 'float32'
   if args.kernel_only:
     kernel_only(torch, args.kernel_only, precision, args.members, args.reps)
+    return
+  if args.workload == 'ensemble':
+    mixed_ensemble(torch, args.steps, args.warmup)
     return
   members = args.members
   print(f'[bench] chosen leg: {GRID} L{LAYERS} moist+3 tracers, {members} members, '

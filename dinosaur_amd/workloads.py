@@ -61,11 +61,13 @@ def humidity_nodal(grid, sigma_centers: np.ndarray, seed: int) -> np.ndarray:
 
 def build_primitive(name: str, layers: int, *, moist: bool = False, members: int = 1,
                     device='cuda', dtype=torch.float32, transform_precision='float32',
-                    impl=None, fused_nodal: bool = True):
+                    impl=None, fused_nodal: bool = True, first_member: int = 0):
   """Returns (equation, state, coords, physics_specs) for the rest-atmosphere
   recipe with seeded orography and perturbations.  With members > 1 the state
   leaves carry a leading member dimension.  `impl` replaces the spherical
-  harmonics implementation (the Grid's plug-in seam)."""
+  harmonics implementation (the Grid's plug-in seam).  Members are numbered
+  from `first_member`, which sets their seeds, so members of several groups
+  of one ensemble get distinct perturbations."""
   import functools  # pylint: disable=g-import-not-at-top
   impl = functools.partial(impl or spherical_harmonic.RealSphericalHarmonics,
                            transform_precision=transform_precision)
@@ -86,8 +88,8 @@ def build_primitive(name: str, layers: int, *, moist: bool = False, members: int
   modal_oro = primitive_equations.truncated_modal_orography(
       torch.as_tensor(aux['orography'], device=device, dtype=torch.float64), coords).to(dtype)
   states = []
-  for mem in range(members):
-    seed = 0 if members == 1 else 100 + mem
+  for mem in range(first_member, first_member + members):
+    seed = 0 if (members == 1 and first_member == 0) else 100 + mem
     dz, dd, dt_ = perturbation_arrays(grid, layers, seed=seed)
     tracers = {}
     if moist:

@@ -301,6 +301,7 @@ def measure_kernel(torch, grid, batch, stage, precision_code, reps=10):
   stream = torch.cuda.current_stream(dev)
   sptr = ctypes.c_void_p(stream.cuda_stream)
   P = lambda t: ctypes.c_void_p(t.data_ptr())
+  plan.apply_tuned(batch, _lib.DINO_F32, precision_code)  # the autotuned schedule, if any
   if stage == 'legendre_inverse':
     call = lambda: lib.dino_legendre_inverse(plan.handle, P(modal), P(fourier), batch, 0,
                                              precision_code, sptr)
@@ -464,6 +465,15 @@ def main():
   print(f'[bench] chosen leg: {GRID} L{LAYERS} moist+3 tracers, {members} members, '
         f'precision={precision}', file=sys.stderr, flush=True)
   eq, state, coords, step, targets = build(torch, precision, members)
+  # launch schedules of the four transform stages, autotuned for both batches
+  from dinosaur_amd import autotune
+  tuned = autotune.tune_grid(coords.horizontal, [members * INV_FIELDS, members * FWD_FIELDS],
+                             reps=3)
+  torch.cuda.empty_cache()
+  tune_summary = {str(b): {st: {'choice': r['choice'],
+                                'ms': {k: round(v, 3) for k, v in r['ms'].items()}}
+                           for st, r in rep.items()} for b, rep in tuned.items()}
+  print(f'[bench] autotune: {tune_summary}', file=sys.stderr, flush=True)
   wall, _ = time_leg(torch, step, state, coords, targets, args.steps, args.warmup,
                      args.forward_only)
   days = DT_SECONDS / 86400.0
@@ -552,7 +562,7 @@ def main():
                  'plain_fp32': plain,
                  'chosen_over_plain': (value / plain['value']) if plain else None,
                  'forward_only': args.forward_only,
-                 'graphed_forward_ms_per_step': graphed_ms,
+                 'graphed_forward_ms_per_step': graphed_ms, 'autotune': tune_summary,
                  'accuracy_vs_fp64': acc},
       'roofline': roof,
       'cpu_baseline': cpu,

@@ -18,6 +18,11 @@ LIB_PATH = os.path.join(_HERE, '_lib', 'libdinosaur_amd.so')
 DINO_OK, DINO_EINVAL, DINO_ESHAPE, DINO_EHIP, DINO_ENOMEM = range(5)
 DINO_F32, DINO_F64 = 0, 1
 PREC_NATIVE, PREC_F16X3, PREC_BF16X3 = 0, 1, 2
+# transform stages and launch schedules (dino_plan_set_schedule)
+STAGE_LEGENDRE_INVERSE, STAGE_LEGENDRE_FORWARD, STAGE_DFT_INVERSE, STAGE_DFT_FORWARD = 0, 1, 2, 3
+SCHED_AUTO, SCHED_ROWS, SCHED_TILED, SCHED_TILED_NARROW = 0, 1, 2, 3
+SCHED_PERSISTENT, SCHED_PER_TILE = 1, 2
+SCHED_MPAIRED = 0x100
 PRECISIONS = {'native': PREC_NATIVE, 'float32': PREC_NATIVE, 'float64': PREC_NATIVE,
               'f16x3': PREC_F16X3, 'bf16x3': PREC_BF16X3}
 
@@ -37,6 +42,8 @@ SIGNATURES = {
     'dino_plan_nodes': (_c_i, [_c_p, _c_dp, _c_dp]),
     'dino_plan_table_size': (_c_i64, [_c_p]),
     'dino_plan_table': (_c_i, [_c_p, _c_dp]),
+    'dino_plan_set_schedule': (_c_i, [_c_p, _c_i, _c_i]),
+    'dino_plan_get_schedule': (_c_i, [_c_p, _c_i]),
     'dino_sht_workspace_bytes': (ctypes.c_size_t, [_c_p, _c_i64, _c_i]),
     'dino_sht_inverse': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
     'dino_sht_forward': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),

@@ -622,7 +622,8 @@ int launch_fft_n(bool forward, const dino_plan* pl, const T* in, T* out, const T
   if constexpr (sizeof(T) == 4 && N >= 128 && N <= 512) {
     const bool aligned = reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
                          reinterpret_cast<uintptr_t>(out) % 16 == 0;
-    if (pl->J % 32 == 0 && aligned && 2 * pl->M - 1 <= N)
+    const int fam = pl->sched[forward ? DINO_STAGE_DFT_FORWARD : DINO_STAGE_DFT_INVERSE] & 0xff;
+    if (fam != DINO_SCHED_PER_TILE && pl->J % 32 == 0 && aligned && 2 * pl->M - 1 <= N)
       return launch_fft_persistent<N>(forward, pl, in, out, tw, qw, B, weighted, s);
   }
   int P = fft_pairs_for<T>(N);

@@ -74,7 +74,17 @@ struct dino_plan {
   int64_t* itile_off;   // (M) first tile of each m, in 16-byte units
   float* ifrag32;       // FP32 fragments
   float* ifragbf;       // BF16 hi/lo fragments (16-byte slots)
+  // Launch schedule per stage (DINO_STAGE_*): DINO_SCHED_* family in the low
+  // byte, DINO_SCHED_MPAIRED flag.  Zero (auto) after creation.
+  int sched[4];
 };
+
+// Zonal wavenumber handled by grid row y of a Legendre launch over M rows:
+// natural order (longest triangular slab first) or paired (0, M-1, 1, M-2,
+// ...), which alternates long and short slabs.
+__device__ __forceinline__ int m_of_block(int y, int M, int paired) {
+  return paired ? ((y & 1) ? M - 1 - (y >> 1) : (y >> 1)) : y;
+}
 
 struct dino_implicit {
   int K, L, R;

@@ -65,7 +65,7 @@ template <bool BF, int NT, bool VOUT>
 __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
     const float* __restrict__ modal, float* __restrict__ fourier,
     const float4* __restrict__ itab, const int64_t* __restrict__ itile_off, int B, int L,
-    int J, int Jh, int Rrows) {
+    int J, int Jh, int Rrows, int mpair) {
   constexpr int TF4 = inv_tile_f4(NT);
   constexpr int PER = (TF4 + 255) / 256;
   constexpr int OPITCH = 16 * NT + 4;
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
   constexpr int MAIN4 = (2 * TF4 > OUT4) ? 2 * TF4 : OUT4;
   __shared__ floatx4 lds[MAIN4];
   __shared__ float astg[4 * 16 * kAPitch];
-  const int m = blockIdx.y;
+  const int m = m_of_block(blockIdx.y, gridDim.y, mpair);
   const int nrows = (m == 0) ? B : 2 * B;
   const int row0 = blockIdx.x * kRows;
   if (row0 >= nrows) return;
@@ -254,12 +254,13 @@ int launch_inv_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
   const dim3 grid((unsigned)((2 * B + kRows - 1) / kRows), (unsigned)pl->M);
   const float4* tab = BF ? reinterpret_cast<const float4*>(pl->ifragbf)
                          : reinterpret_cast<const float4*>(pl->ifrag32);
+  const int mpair = (pl->sched[DINO_STAGE_LEGENDRE_INVERSE] & DINO_SCHED_MPAIRED) ? 1 : 0;
   if (vout)
     leg_inv_rows_kernel<BF, NT, true><<<grid, 256, 0, s>>>(in, out, tab, pl->itile_off, (int)B,
-                                                           pl->L, pl->J, pl->Jh, pl->R);
+                                                           pl->L, pl->J, pl->Jh, pl->R, mpair);
   else
     leg_inv_rows_kernel<BF, NT, false><<<grid, 256, 0, s>>>(in, out, tab, pl->itile_off, (int)B,
-                                                            pl->L, pl->J, pl->Jh, pl->R);
+                                                            pl->L, pl->J, pl->Jh, pl->R, mpair);
   DINO_LAUNCH_CHECK("legendre inverse rows kernel");
   return DINO_OK;
 }

@@ -97,13 +97,14 @@ struct PolicySplit {
 template <class Pol, int TC, int KH>
 __global__ __launch_bounds__(kThreads) void leg_inv_mfma_kernel(
     const float* __restrict__ modal, float* __restrict__ fourier, const float* __restrict__ ptab,
-    const int64_t* __restrict__ m_off, int B, int L, int J, int Jh, int Jhp, int Rrows) {
+    const int64_t* __restrict__ m_off, int B, int L, int J, int Jh, int Jhp, int Rrows,
+    int mpair) {
   constexpr int WR = TR / 2, WC = TC / 2, MT = WR / 16, NT = WC / 16;
   constexpr int KC = 2 * KH;
   constexpr int PA = TR + 4, PB = TC + 4;
   constexpr int A_PER = TR * KC / kThreads, P_PER = KC * TC / kThreads;
   static_assert(TR * KC % kThreads == 0 && KC * TC % kThreads == 0, "tile / thread mismatch");
-  const int m = blockIdx.z;
+  const int m = m_of_block(blockIdx.z, gridDim.z, mpair);
   const int nrows = (m == 0) ? B : 2 * B;
   const int row0 = blockIdx.y * TR;
   if (row0 >= nrows) return;
@@ -207,13 +208,14 @@ __global__ __launch_bounds__(kThreads) void leg_inv_mfma_kernel(
 template <class Pol, int KC>
 __global__ __launch_bounds__(kThreads) void leg_fwd_mfma_kernel(
     const float* __restrict__ fourier, float* __restrict__ modal, const float* __restrict__ ptab,
-    const int64_t* __restrict__ m_off, int B, int L, int J, int Jh, int Jhp, int Rrows) {
+    const int64_t* __restrict__ m_off, int B, int L, int J, int Jh, int Jhp, int Rrows,
+    int mpair) {
   constexpr int TLH = 64;  // degrees per parity in the tile
   constexpr int WR = TR / 2, WC = TLH / 2, MT = WR / 16, NT = WC / 16;
   constexpr int PA = TR + 4, PB = TLH + 4;
   constexpr int S_PER = TR * KC / kThreads, P_PER = 2 * TLH * KC / kThreads;
   static_assert(TR * KC % kThreads == 0 && 2 * TLH * KC % kThreads == 0, "tile mismatch");
-  const int m = blockIdx.z;
+  const int m = m_of_block(blockIdx.z, gridDim.z, mpair);
   const int nrows = (m == 0) ? B : 2 * B;
   const int row0 = blockIdx.y * TR;
   if (row0 >= nrows) return;
@@ -332,8 +334,9 @@ namespace {
 template <class Pol, int TC, int KH>
 int launch_inv(const dino_plan* pl, const float* modal, float* fourier, int64_t B, hipStream_t s) {
   dim3 grid((pl->Jh + TC - 1) / TC, (unsigned)((2 * B + TR - 1) / TR), pl->M);
+  const int mpair = (pl->sched[DINO_STAGE_LEGENDRE_INVERSE] & DINO_SCHED_MPAIRED) ? 1 : 0;
   leg_inv_mfma_kernel<Pol, TC, KH><<<grid, kThreads, 0, s>>>(
-      modal, fourier, pl->p32, pl->m_offset, (int)B, pl->L, pl->J, pl->Jh, pl->Jhp, pl->R);
+      modal, fourier, pl->p32, pl->m_offset, (int)B, pl->L, pl->J, pl->Jh, pl->Jhp, pl->R, mpair);
   DINO_LAUNCH_CHECK("legendre inverse mfma kernel");
   return DINO_OK;
 }
@@ -341,16 +344,22 @@ int launch_inv(const dino_plan* pl, const float* modal, float* fourier, int64_t 
 template <class Pol, int KC>
 int launch_fwd(const dino_plan* pl, const float* fourier, float* modal, int64_t B, hipStream_t s) {
   dim3 grid((pl->L + 127) / 128, (unsigned)((2 * B + TR - 1) / TR), pl->M);
+  const int mpair = (pl->sched[DINO_STAGE_LEGENDRE_FORWARD] & DINO_SCHED_MPAIRED) ? 1 : 0;
   leg_fwd_mfma_kernel<Pol, KC><<<grid, kThreads, 0, s>>>(
-      fourier, modal, pl->p32, pl->m_offset, (int)B, pl->L, pl->J, pl->Jh, pl->Jhp, pl->R);
+      fourier, modal, pl->p32, pl->m_offset, (int)B, pl->L, pl->J, pl->Jh, pl->Jhp, pl->R, mpair);
   DINO_LAUNCH_CHECK("legendre forward mfma kernel");
   return DINO_OK;
 }
 
+// Node-tile width by latitude count; `narrow` halves the widest tile, which
+// doubles the workgroups per (rows, m) at the cost of re-reading the modal
+// rows once more.
 template <class Pol, int KH>
-int inv_by_width(const dino_plan* pl, const float* in, float* out, int64_t B, hipStream_t s) {
-  if (pl->Jh > 64) return launch_inv<Pol, (KH > 16 ? 64 : 128), KH>(pl, in, out, B, s);
-  if (pl->Jh > 32) return launch_inv<Pol, 64, KH>(pl, in, out, B, s);
+int inv_by_width(const dino_plan* pl, const float* in, float* out, int64_t B, bool narrow,
+                 hipStream_t s) {
+  if (pl->Jh > 64 && !narrow) return launch_inv<Pol, (KH > 16 ? 64 : 128), KH>(pl, in, out, B, s);
+  if (pl->Jh > 32 && !narrow) return launch_inv<Pol, 64, KH>(pl, in, out, B, s);
+  if (pl->Jh > 64) return launch_inv<Pol, (KH > 16 ? 32 : 64), KH>(pl, in, out, B, s);
   return launch_inv<Pol, 32, KH>(pl, in, out, B, s);
 }
 
@@ -366,12 +375,15 @@ int legendre_inverse_mfma(const dino_plan* pl, const void* modal, void* fourier,
   if (B == 0) return DINO_OK;
   const float* in = static_cast<const float*>(modal);
   float* out = static_cast<float*>(fourier);
-  if (pl->int_nt > 0 && (precision == DINO_PREC_NATIVE || precision == DINO_PREC_BF16X3))
+  const int fam = pl->sched[DINO_STAGE_LEGENDRE_INVERSE] & 0xff;
+  if (pl->int_nt > 0 && (fam == DINO_SCHED_AUTO || fam == DINO_SCHED_ROWS) &&
+      (precision == DINO_PREC_NATIVE || precision == DINO_PREC_BF16X3))
     return legendre_inverse_rows(pl, in, out, B, precision == DINO_PREC_BF16X3, s);
   DINO_REQUIRE((2 * B + TR - 1) / TR <= 65535, DINO_EINVAL, "batch too large");
+  const bool narrow = fam == DINO_SCHED_TILED_NARROW;
   switch (precision) {
-    case DINO_PREC_NATIVE: return inv_by_width<PolicyF32, 16>(pl, in, out, B, s);
-    case DINO_PREC_BF16X3: return inv_by_width<PolicySplit<true>, 32>(pl, in, out, B, s);
+    case DINO_PREC_NATIVE: return inv_by_width<PolicyF32, 16>(pl, in, out, B, narrow, s);
+    case DINO_PREC_BF16X3: return inv_by_width<PolicySplit<true>, 32>(pl, in, out, B, narrow, s);
     default:
       set_error("split precision %d is not available; use DINO_PREC_BF16X3", precision);
       return DINO_EINVAL;
@@ -383,11 +395,16 @@ int legendre_forward_mfma(const dino_plan* pl, const void* fourier, void* modal,
   if (B == 0) return DINO_OK;
   const float* in = static_cast<const float*>(fourier);
   float* out = static_cast<float*>(modal);
-  if (pl->fks > 0 && (precision == DINO_PREC_NATIVE || precision == DINO_PREC_BF16X3))
+  const int fam = pl->sched[DINO_STAGE_LEGENDRE_FORWARD] & 0xff;
+  if (pl->fks > 0 && (fam == DINO_SCHED_AUTO || fam == DINO_SCHED_ROWS) &&
+      (precision == DINO_PREC_NATIVE || precision == DINO_PREC_BF16X3))
     return legendre_forward_rows(pl, in, out, B, precision == DINO_PREC_BF16X3, s);
   DINO_REQUIRE((2 * B + TR - 1) / TR <= 65535, DINO_EINVAL, "batch too large");
   switch (precision) {
-    case DINO_PREC_NATIVE: return launch_fwd<PolicyF32, 32>(pl, in, out, B, s);
+    case DINO_PREC_NATIVE:
+      // narrow: 16 nodes per reduction chunk (half the LDS, more resident workgroups)
+      if (fam == DINO_SCHED_TILED_NARROW) return launch_fwd<PolicyF32, 16>(pl, in, out, B, s);
+      return launch_fwd<PolicyF32, 32>(pl, in, out, B, s);
     case DINO_PREC_BF16X3: return launch_fwd<PolicySplit<true>, 32>(pl, in, out, B, s);
     default:
       set_error("split precision %d is not available; use DINO_PREC_BF16X3", precision);

@@ -81,7 +81,7 @@ template <bool BF, int KS, bool VIN, bool VOUT>
 __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
     const float* __restrict__ fourier, float* __restrict__ modal,
     const float4* __restrict__ ftab, const int64_t* __restrict__ ftile_off, int B, int L,
-    int J, int Jh, int Rrows) {
+    int J, int Jh, int Rrows, int mpair) {
   constexpr int KT = 2 * KS;  // 16-node groups
   constexpr int TF4 = tile_f4(KS);
   constexpr int PER = TF4 / 256;
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
   constexpr int LDS4 = (2 * TF4 > STG4) ? 2 * TF4 : STG4;
   __shared__ floatx4 lds[LDS4];
   __shared__ float ostage[VOUT ? 1 : 4 * 16 * kOutPitch];
-  const int m = blockIdx.y;
+  const int m = m_of_block(blockIdx.y, gridDim.y, mpair);
   const int nrows = (m == 0) ? B : 2 * B;
   const int row0 = blockIdx.x * kRowsPerBlock;
   if (row0 >= nrows) return;
@@ -358,9 +358,10 @@ int launch_rows(const dino_plan* pl, const float* in, float* out, int64_t B, hip
   const dim3 grid((unsigned)((2 * B + kRowsPerBlock - 1) / kRowsPerBlock), (unsigned)pl->M);
   const float4* tab = BF ? reinterpret_cast<const float4*>(pl->ffragbf)
                          : reinterpret_cast<const float4*>(pl->ffrag32);
+  const int mpair = (pl->sched[DINO_STAGE_LEGENDRE_FORWARD] & DINO_SCHED_MPAIRED) ? 1 : 0;
 #define ROWS_LAUNCH(VI, VO)                                                                  \
   leg_fwd_rows_kernel<BF, KS, VI, VO><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, (int)B, \
-                                                            pl->L, pl->J, pl->Jh, pl->R)
+                                                            pl->L, pl->J, pl->Jh, pl->R, mpair)
   if (vin && vout) ROWS_LAUNCH(true, true);
   else if (vin) ROWS_LAUNCH(true, false);
   else if (vout) ROWS_LAUNCH(false, true);

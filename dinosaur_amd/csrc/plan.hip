@@ -323,6 +323,37 @@ extern "C" int dino_plan_destroy(dino_plan* plan) {
   return DINO_OK;
 }
 
+extern "C" int dino_plan_set_schedule(dino_plan* p, int stage, int schedule) {
+  DINO_REQUIRE(p, DINO_EINVAL, "dino_plan_set_schedule: NULL plan");
+  DINO_REQUIRE(stage >= DINO_STAGE_LEGENDRE_INVERSE && stage <= DINO_STAGE_DFT_FORWARD,
+               DINO_EINVAL, "dino_plan_set_schedule: invalid stage %d", stage);
+  const int fam = schedule & 0xff, flags = schedule & ~0xff;
+  const bool legendre = stage <= DINO_STAGE_LEGENDRE_FORWARD;
+  DINO_REQUIRE(schedule >= 0 && (flags & ~DINO_SCHED_MPAIRED) == 0 &&
+                   (legendre || flags == 0),
+               DINO_EINVAL, "dino_plan_set_schedule: invalid flags in schedule 0x%x", schedule);
+  if (legendre) {
+    DINO_REQUIRE(fam <= DINO_SCHED_TILED_NARROW, DINO_EINVAL,
+                 "dino_plan_set_schedule: invalid Legendre kernel family %d", fam);
+    if (fam == DINO_SCHED_ROWS) {
+      const bool have = stage == DINO_STAGE_LEGENDRE_INVERSE ? p->int_nt > 0 : p->fks > 0;
+      DINO_REQUIRE(have, DINO_EINVAL,
+                   "dino_plan_set_schedule: no row-resident Legendre kernel for %d latitudes",
+                   p->J);
+    }
+  } else {
+    DINO_REQUIRE(fam <= DINO_SCHED_PER_TILE, DINO_EINVAL,
+                 "dino_plan_set_schedule: invalid DFT kernel family %d", fam);
+  }
+  p->sched[stage] = schedule;
+  return DINO_OK;
+}
+
+extern "C" int dino_plan_get_schedule(const dino_plan* p, int stage) {
+  if (!p || stage < DINO_STAGE_LEGENDRE_INVERSE || stage > DINO_STAGE_DFT_FORWARD) return -1;
+  return p->sched[stage];
+}
+
 extern "C" int dino_plan_nodes(const dino_plan* p, double* nodes, double* weights) {
   DINO_REQUIRE(p && nodes && weights, DINO_EINVAL, "dino_plan_nodes: NULL argument");
   DINO_HIP(hipMemcpy(nodes, p->nodes64, sizeof(double) * p->J, hipMemcpyDeviceToHost));

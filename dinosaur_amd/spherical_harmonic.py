@@ -113,6 +113,8 @@ class Plan:
         rc = lib.dino_plan_create(M, L, I, J, _lib.dptr(x), _lib.dptr(w), ctypes.byref(handle))
     _lib.check(rc, 'dino_plan_create')
     self.handle = handle
+    # autotuned schedules: (batch bucket, dtype code, precision) -> {stage: schedule}
+    self.tuned = {}
 
   def __del__(self):
     try:
@@ -127,6 +129,26 @@ class Plan:
     w = np.empty(self.J)
     _lib.check(_lib.lib().dino_plan_nodes(self.handle, _lib.dptr(x), _lib.dptr(w)))
     return x, w
+
+  def set_schedule(self, stage: int, schedule: int):
+    """Kernel family (+ flags) for one transform stage (_lib.STAGE_*, SCHED_*)."""
+    _lib.check(_lib.lib().dino_plan_set_schedule(self.handle, stage, schedule),
+               'dino_plan_set_schedule')
+
+  def schedule(self, stage: int) -> int:
+    return _lib.lib().dino_plan_get_schedule(self.handle, stage)
+
+  def apply_tuned(self, batch: int, dtype_code: int, precision: int):
+    """Sets the autotuned schedules for this batch size, if any were tuned:
+    the entry of the same power-of-two batch bucket, else the nearest one."""
+    if not self.tuned:
+      return
+    from . import autotune  # pylint: disable=g-import-not-at-top
+    entry = autotune.lookup(self.tuned, batch, dtype_code, precision)
+    if entry is not None:
+      for stage, sched in entry.items():
+        if _lib.lib().dino_plan_get_schedule(self.handle, stage) != sched:
+          self.set_schedule(stage, sched)
 
   def folded_table(self) -> np.ndarray:
     n = _lib.lib().dino_plan_table_size(self.handle)
@@ -184,6 +206,7 @@ def sht_call(fn_name: str, plan: Plan, x: torch.Tensor, out_tail: tuple, precisi
   work = _workspace(plan, batch, x)
   if x.dtype == torch.float64:
     precision = _lib.PREC_NATIVE  # FP64 data always runs FP64 arithmetic
+  plan.apply_tuned(batch, _dtype_code(x), precision)
   fn = getattr(_lib.lib(), fn_name)
   with torch.cuda.device(x.device):
     rc = fn(plan.handle, _ptr(x), _ptr(out), _ptr(work), batch, _dtype_code(x), precision,

@@ -80,6 +80,35 @@ int dino_plan_nodes(const dino_plan* plan, double* nodes, double* weights);
 int64_t dino_plan_table_size(const dino_plan* plan);
 int dino_plan_table(const dino_plan* plan, double* out);
 
+/* Launch schedules of the four transform stages.  A schedule is a kernel
+ * family in the low byte plus flags; every family computes the same
+ * contraction (FP32 data; FP64 data always runs the FP64 kernels), so a
+ * schedule changes speed and FP32 rounding order, never the result beyond
+ * rounding.  The project's autotuner (dinosaur_amd/autotune.py) times the
+ * candidates for a batch size and sets the fastest.  Set schedules before
+ * launching work that uses the plan: the plan is read at launch time, so a
+ * change races with transforms running concurrently on other threads.  The
+ * reference has no counterpart (XLA picks its einsum kernels internally). */
+enum {
+  DINO_STAGE_LEGENDRE_INVERSE = 0,
+  DINO_STAGE_LEGENDRE_FORWARD = 1,
+  DINO_STAGE_DFT_INVERSE = 2,
+  DINO_STAGE_DFT_FORWARD = 3,
+};
+enum {
+  DINO_SCHED_AUTO = 0,         /* built-in choice (row-resident / persistent where possible) */
+  DINO_SCHED_ROWS = 1,         /* Legendre: row-resident kernel, all degree tiles of one m per workgroup */
+  DINO_SCHED_TILED = 2,        /* Legendre: (rows x nodes/degrees) tiled kernel, wide tiles */
+  DINO_SCHED_TILED_NARROW = 3, /* Legendre: tiled kernel, narrow tiles (more workgroups) */
+  DINO_SCHED_PERSISTENT = 1,   /* DFT: persistent FP32 FFT workgroups (one wave per CU) */
+  DINO_SCHED_PER_TILE = 2,     /* DFT: one workgroup per (field, latitude tile) */
+  DINO_SCHED_MPAIRED = 0x100,  /* Legendre flag: m order 0, M-1, 1, M-2, ... (long and
+                                  short triangular slabs alternate) instead of 0, 1, 2, ... */
+};
+int dino_plan_set_schedule(dino_plan* plan, int stage, int schedule);
+/* Current schedule of a stage (>= 0), or -1 for an invalid plan or stage. */
+int dino_plan_get_schedule(const dino_plan* plan, int stage);
+
 /* Bytes of workspace the transforms below need for `batch` fields. */
 size_t dino_sht_workspace_bytes(const dino_plan* plan, int64_t batch,
                                 int dtype);

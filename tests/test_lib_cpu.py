@@ -43,3 +43,22 @@ def test_host_gauss_nodes(n):
 def test_error_mapping():
   with pytest.raises(ValueError):
     _lib.check(_lib.lib().dino_gauss_nodes_host(0, None, None))
+
+
+def test_schedule_entry_points_reject_invalid_arguments():
+  lib = _lib.lib()
+  assert lib.dino_plan_set_schedule(None, _lib.STAGE_LEGENDRE_INVERSE, 0) == _lib.DINO_EINVAL
+  assert b'NULL plan' in lib.dino_last_error()
+  assert lib.dino_plan_get_schedule(None, 0) == -1
+
+
+def test_autotune_batch_buckets_and_lookup():
+  from dinosaur_amd import autotune
+  assert [autotune.bucket(b) for b in (1, 2, 3, 64, 65, 5984, 10672)] == \
+      [1, 2, 4, 64, 128, 8192, 16384]
+  tuned = {(8192, 0, 0): {0: 1}, (16384, 0, 0): {0: 2}, (64, 0, 2): {0: 3}}
+  assert autotune.lookup(tuned, 5000, 0, 0) == {0: 1}
+  assert autotune.lookup(tuned, 12000, 0, 0) == {0: 2}
+  assert autotune.lookup(tuned, 10 ** 6, 0, 0) == {0: 2}   # nearest bucket
+  assert autotune.lookup(tuned, 100, 0, 2) == {0: 3}       # other precision
+  assert autotune.lookup(tuned, 100, 1, 0) is None         # nothing tuned for FP64

@@ -17,9 +17,9 @@
 namespace dino {
 
 template <typename T>
-int implicit_mfma_dispatch(bool solve, bool adjoint, const dino_implicit* f, const void* div,
-                           const void* temp, const void* lnps, void* d_o, void* t_o, void* l_o,
-                           int64_t members, hipStream_t s);
+int implicit_mfma_dispatch(bool solve, bool adjoint, bool acc, const dino_implicit* f,
+                           const void* div, const void* temp, const void* lnps, void* d_o,
+                           void* t_o, void* l_o, int64_t members, hipStream_t s);
 int implicit_mfma_tables(int K, const double* G, const double* H, const double* tref,
                          const double* dsigma, const double* V, const double* Vinv,
                          const double* d, double r_gas, std::vector<double>* out);
@@ -28,13 +28,13 @@ namespace {
 
 constexpr int kTables = 15;
 
-int run(bool solve, bool adjoint, const dino_implicit* f, const void* div, const void* temp,
-        const void* lnps, void* d_o, void* t_o, void* l_o, int64_t members, int dtype,
-        void* stream) {
+int run(bool solve, bool adjoint, bool acc, const dino_implicit* f, const void* div,
+        const void* temp, const void* lnps, void* d_o, void* t_o, void* l_o, int64_t members,
+        int dtype, void* stream) {
   if (dtype == DINO_F64)
-    return implicit_mfma_dispatch<double>(solve, adjoint, f, div, temp, lnps, d_o, t_o, l_o,
-                                          members, as_stream(stream));
-  return implicit_mfma_dispatch<float>(solve, adjoint, f, div, temp, lnps, d_o, t_o, l_o,
+    return implicit_mfma_dispatch<double>(solve, adjoint, acc, f, div, temp, lnps, d_o, t_o,
+                                          l_o, members, as_stream(stream));
+  return implicit_mfma_dispatch<float>(solve, adjoint, acc, f, div, temp, lnps, d_o, t_o, l_o,
                                        members, as_stream(stream));
 }
 
@@ -120,7 +120,8 @@ extern "C" int dino_implicit_destroy(dino_implicit* f) {
 
 static int checked(const char* name, bool solve, bool adjoint, const dino_implicit* f,
                    const void* div, const void* temp, const void* lnps, void* div_out,
-                   void* temp_out, void* lnps_out, int64_t members, int dtype, void* stream) {
+                   void* temp_out, void* lnps_out, int64_t members, int dtype, void* stream,
+                   bool acc = false) {
   DINO_REQUIRE(f && div && temp && lnps && div_out && temp_out && lnps_out, DINO_EINVAL,
                "%s: NULL argument", name);
   DINO_REQUIRE(!solve || (f->V && f->Vinv && f->d), DINO_EINVAL,
@@ -128,8 +129,8 @@ static int checked(const char* name, bool solve, bool adjoint, const dino_implic
   DINO_REQUIRE(members >= 0, DINO_EINVAL, "%s: negative member count", name);
   DINO_REQUIRE(dtype == DINO_F32 || dtype == DINO_F64, DINO_EINVAL, "%s: invalid dtype %d",
                name, dtype);
-  return run(solve, adjoint, f, div, temp, lnps, div_out, temp_out, lnps_out, members, dtype,
-             stream);
+  return run(solve, adjoint, acc, f, div, temp, lnps, div_out, temp_out, lnps_out, members,
+             dtype, stream);
 }
 
 extern "C" int dino_implicit_solve(const dino_implicit* f, const void* div, const void* temp,
@@ -160,4 +161,11 @@ extern "C" int dino_implicit_terms_adjoint(const dino_implicit* f, const void* d
                                            int64_t members, int dtype, void* stream) {
   return checked("dino_implicit_terms_adjoint", false, true, f, div_cot, temp_cot, lnps_cot,
                  div_out, temp_out, lnps_out, members, dtype, stream);
+}
+
+extern "C" int dino_implicit_terms_adjoint_accumulate(
+    const dino_implicit* f, const void* div_cot, const void* temp_cot, const void* lnps_cot,
+    void* div_out, void* temp_out, void* lnps_out, int64_t members, int dtype, void* stream) {
+  return checked("dino_implicit_terms_adjoint_accumulate", false, true, f, div_cot, temp_cot,
+                 lnps_cot, div_out, temp_out, lnps_out, members, dtype, stream, true);
 }

@@ -58,10 +58,11 @@ __device__ __forceinline__ double ldd(const T* p) {
 //   delta_bar = W^T z_bar, T_bar = dT - eta lam (W G)^T z_bar,
 //   lnps_bar = dp - eta lam w . z_bar
 // -- the tables are swapped by the host and eta / eta lam trade places.
+// In place is allowed (outputs equal to inputs): a wave reads every level of
+// its 16 columns before it writes any, and no other wave touches them.
 template <typename T, int NS, bool TRANS>
 __global__ __launch_bounds__(256) void solve_mfma_kernel(
-    const T* __restrict__ div, const T* __restrict__ temp, const T* __restrict__ lnps,
-    T* __restrict__ div_out, T* __restrict__ temp_out, T* __restrict__ lnps_out,
+    const T* div, const T* temp, const T* lnps, T* div_out, T* temp_out, T* lnps_out,
     const double* __restrict__ t1a, const double* __restrict__ t1b,
     const double* __restrict__ t2, const double* __restrict__ wvec,
     const double* __restrict__ dvec, const double* __restrict__ lam, int K, int L,
@@ -144,10 +145,11 @@ __global__ __launch_bounds__(256) void solve_mfma_kernel(
 //   delta_bar = -(H^T dT + dsigma dp),  T_bar = -lam G^T dd,
 //   lnps_bar = -lam (R T_ref) . dd
 // with tg = H^T, th = [G^T; (R T_ref)^T], v1 = dsigma (host-side swap).
-template <typename T, int NS, bool TRANS>
+// ACC: add the result to the outputs instead of overwriting them.  In place
+// is allowed, as for the solve.
+template <typename T, int NS, bool TRANS, bool ACC>
 __global__ __launch_bounds__(256) void terms_mfma_kernel(
-    const T* __restrict__ div, const T* __restrict__ temp, const T* __restrict__ lnps,
-    T* __restrict__ div_out, T* __restrict__ temp_out, T* __restrict__ lnps_out,
+    const T* div, const T* temp, const T* lnps, T* div_out, T* temp_out, T* lnps_out,
     const double* __restrict__ tg, const double* __restrict__ th,
     const double* __restrict__ rtref, const double* __restrict__ lam, int K, int L,
     int64_t cols, int64_t members) {
@@ -184,8 +186,11 @@ __global__ __launch_bounds__(256) void terms_mfma_kernel(
     for (int i = 0; i < 4; ++i) {
       const int row = 16 * it + 4 * i + kg;
       const double s1 = TRANS ? 1.0 : lam_l;
-      if (cok && row < K)
-        div_out[base + (int64_t)row * cols] = (T)(-s1 * (acc[i] + rtref[row] * lp));
+      if (cok && row < K) {
+        T* o = div_out + base + (int64_t)row * cols;
+        const T v = (T)(-s1 * (acc[i] + rtref[row] * lp));
+        *o = ACC ? (T)(*o + v) : v;
+      }
     }
   }
   // temperature and log-pressure tendencies: -[H; dsigma] delta
@@ -200,13 +205,21 @@ __global__ __launch_bounds__(256) void terms_mfma_kernel(
     for (int i = 0; i < 4; ++i) {
       const int row = 16 * it + 4 * i + kg;
       if (row < K) {
-        if (cok) temp_out[base + (int64_t)row * cols] = (T)(-s2 * acc[i]);
+        if (cok) {
+          T* o = temp_out + base + (int64_t)row * cols;
+          const T v = (T)(-s2 * acc[i]);
+          *o = ACC ? (T)(*o + v) : v;
+        }
       } else if (row == S::KP) {
         sacc = acc[i];
       }
     }
   }
-  if (kg == (S::KP & 3) && cok) lnps_out[b * cols + cc] = (T)(-s2 * sacc);
+  if (kg == (S::KP & 3) && cok) {
+    T* o = lnps_out + b * cols + cc;
+    const T v = (T)(-s2 * sacc);
+    *o = ACC ? (T)(*o + v) : v;
+  }
 }
 
 }  // namespace
@@ -302,9 +315,9 @@ int implicit_mfma_tables(int K, const double* G, const double* H, const double* 
 }
 
 template <typename T, int NS>
-static int launch_ns(bool solve, bool adjoint, const dino_implicit* f, const void* div,
-                     const void* temp, const void* lnps, void* d_o, void* t_o, void* l_o,
-                     int64_t members, hipStream_t s) {
+static int launch_ns(bool solve, bool adjoint, bool acc, const dino_implicit* f,
+                     const void* div, const void* temp, const void* lnps, void* d_o, void* t_o,
+                     void* l_o, int64_t members, hipStream_t s) {
   const int64_t cols = (int64_t)f->R * f->L;
   const int64_t waves = (cols + 15) / 16 * members;
   if (waves == 0) return DINO_OK;
@@ -325,10 +338,13 @@ static int launch_ns(bool solve, bool adjoint, const dino_implicit* f, const voi
         x0, x1, x2, y0, y1, y2, t[8], t[9], t[10], t[11], t[4], f->lam, f->K, f->L, cols,
         members, f->eta);
   } else if (!adjoint) {
-    terms_mfma_kernel<T, NS, false><<<blocks, 256, 0, s>>>(
+    terms_mfma_kernel<T, NS, false, false><<<blocks, 256, 0, s>>>(
         x0, x1, x2, y0, y1, y2, t[5], t[6], t[7], f->lam, f->K, f->L, cols, members);
+  } else if (!acc) {
+    terms_mfma_kernel<T, NS, true, false><<<blocks, 256, 0, s>>>(
+        x0, x1, x2, y0, y1, y2, t[12], t[13], t[14], f->lam, f->K, f->L, cols, members);
   } else {
-    terms_mfma_kernel<T, NS, true><<<blocks, 256, 0, s>>>(
+    terms_mfma_kernel<T, NS, true, true><<<blocks, 256, 0, s>>>(
         x0, x1, x2, y0, y1, y2, t[12], t[13], t[14], f->lam, f->K, f->L, cols, members);
   }
   DINO_LAUNCH_CHECK("implicit mfma kernel");
@@ -336,13 +352,13 @@ static int launch_ns(bool solve, bool adjoint, const dino_implicit* f, const voi
 }
 
 template <typename T>
-int implicit_mfma_dispatch(bool solve, bool adjoint, const dino_implicit* f, const void* div,
-                           const void* temp, const void* lnps, void* d_o, void* t_o, void* l_o,
-                           int64_t members, hipStream_t s) {
+int implicit_mfma_dispatch(bool solve, bool adjoint, bool acc, const dino_implicit* f,
+                           const void* div, const void* temp, const void* lnps, void* d_o,
+                           void* t_o, void* l_o, int64_t members, hipStream_t s) {
   switch (implicit_mfma_steps(f->K)) {
 #define NS_CASE(n) \
   case n:          \
-    return launch_ns<T, n>(solve, adjoint, f, div, temp, lnps, d_o, t_o, l_o, members, s);
+    return launch_ns<T, n>(solve, adjoint, acc, f, div, temp, lnps, d_o, t_o, l_o, members, s);
     NS_CASE(1) NS_CASE(2) NS_CASE(3) NS_CASE(4) NS_CASE(5) NS_CASE(6) NS_CASE(7) NS_CASE(8)
     NS_CASE(9) NS_CASE(10) NS_CASE(11) NS_CASE(12) NS_CASE(13) NS_CASE(14) NS_CASE(15)
     NS_CASE(16)
@@ -353,11 +369,11 @@ int implicit_mfma_dispatch(bool solve, bool adjoint, const dino_implicit* f, con
   }
 }
 
-template int implicit_mfma_dispatch<float>(bool, bool, const dino_implicit*, const void*,
+template int implicit_mfma_dispatch<float>(bool, bool, bool, const dino_implicit*, const void*,
                                            const void*, const void*, void*, void*, void*,
                                            int64_t, hipStream_t);
-template int implicit_mfma_dispatch<double>(bool, bool, const dino_implicit*, const void*,
-                                            const void*, const void*, void*, void*, void*,
-                                            int64_t, hipStream_t);
+template int implicit_mfma_dispatch<double>(bool, bool, bool, const dino_implicit*,
+                                            const void*, const void*, const void*, void*, void*,
+                                            void*, int64_t, hipStream_t);
 
 }  // namespace dino

@@ -12,6 +12,11 @@
 //
 // The adjoint reads the output cotangent once and writes the cotangent of
 // every f_j and g_k: fl(a_j fl(dt gbar)).
+//
+// Ranged forms: the implicit terms may be stored only on a sub-range of the
+// state (where the implicit tendency can be nonzero), and the adjoint may
+// accumulate into its targets, so a hand-written reverse sweep of a whole
+// IMEX step needs no separate gradient additions.
 #include "dino_common.h"
 
 // separate roundings of every product and sum, as the elementwise passes do
@@ -36,6 +41,10 @@ struct Vec<double> {
   static constexpr int W = 2;
 };
 
+constexpr int kMaxTargets = 9;
+
+// Implicit terms live on [ib, ie): element i of x[1][k] is x[1][k][i - go].
+// A full-range call has ib = go = 0 and ie = n.
 template <typename T>
 struct StageArgs {
   const T* y0;
@@ -43,6 +52,18 @@ struct StageArgs {
   T c[2][kMaxTerms];
   int n[2];
   T dt;
+  int64_t ib, ie, go;
+};
+
+template <typename T>
+struct TargetArgs {
+  const T* g;
+  T* out[kMaxTargets];
+  T c[kMaxTargets];
+  int kind[kMaxTargets];
+  int n;
+  T dt;
+  int64_t ib, ie, go;
 };
 
 template <typename T>
@@ -54,26 +75,30 @@ struct AdjointArgs {
   T dt;
 };
 
-// U is T (scalar tail) or Vec<T>::type (16-byte body); the arithmetic is
-// componentwise either way.
+// U is T (scalar) or Vec<T>::type (16-byte vector); the arithmetic is
+// componentwise either way.  idx indexes U elements of the group's arrays.
 template <typename T, typename U>
-__device__ __forceinline__ U combine_one(const StageArgs<T>& a, int64_t i) {
+__device__ __forceinline__ U group_sum(const StageArgs<T>& a, int grp, int64_t idx) {
   auto ld = [](const T* p, int64_t k) { return reinterpret_cast<const U*>(p)[k]; };
-  U y = ld(a.y0, i);
+  U s = a.c[grp][0] * ld(a.x[grp][0], idx);
 #pragma unroll
-  for (int grp = 0; grp < 2; ++grp) {
-    if (a.n[grp] == 0) continue;
-    U s = a.c[grp][0] * ld(a.x[grp][0], i);
-#pragma unroll
-    for (int j = 1; j < kMaxTerms; ++j)
-      if (j < a.n[grp]) s = s + a.c[grp][j] * ld(a.x[grp][j], i);
-    y = y + a.dt * s;
-  }
+  for (int j = 1; j < kMaxTerms; ++j)
+    if (j < a.n[grp]) s = s + a.c[grp][j] * ld(a.x[grp][j], idx);
+  return s;
+}
+
+template <typename T>
+__device__ __forceinline__ T combine_scalar(const StageArgs<T>& a, int64_t i) {
+  T y = a.y0[i];
+  if (a.n[0]) y = y + a.dt * group_sum<T, T>(a, 0, i);
+  if (a.n[1] && i >= a.ib && i < a.ie) y = y + a.dt * group_sum<T, T>(a, 1, i - a.go);
   return y;
 }
 
-// VEC: every pointer 16-byte aligned; thread k handles vector k, and the
-// n % W tail is done by scalar threads after the last full vector.
+// VEC: every full-size pointer 16-byte aligned, implicit pointers too, and
+// go a multiple of the vector width; thread k handles vector k (vectors that
+// straddle a range end go component by component), and the n % W tail is done
+// by scalar threads after the last full vector.
 template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a, T* __restrict__ out,
                                                     int64_t n) {
@@ -82,11 +107,28 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a, T* __restric
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nv = n / W;
   if (k < nv) {
-    if constexpr (VEC) reinterpret_cast<V*>(out)[k] = combine_one<T, V>(a, k);
-    else out[k] = combine_one<T, T>(a, k);
+    if constexpr (VEC) {
+      const int64_t i0 = k * W;
+      V y = reinterpret_cast<const V*>(a.y0)[k];
+      if (a.n[0]) y = y + a.dt * group_sum<T, V>(a, 0, k);
+      if (a.n[1]) {
+        if (i0 >= a.ib && i0 + W <= a.ie) {
+          y = y + a.dt * group_sum<T, V>(a, 1, (i0 - a.go) / W);
+        } else if (i0 + W > a.ib && i0 < a.ie) {
+#pragma unroll
+          for (int e = 0; e < W; ++e) {
+            const int64_t i = i0 + e;
+            if (i >= a.ib && i < a.ie) y[e] = y[e] + a.dt * group_sum<T, T>(a, 1, i - a.go);
+          }
+        }
+      }
+      reinterpret_cast<V*>(out)[k] = y;
+    } else {
+      out[k] = combine_scalar(a, k);
+    }
   } else if (VEC && k - nv < n - nv * W) {
     const int64_t i = nv * W + (k - nv);
-    out[i] = combine_one<T, T>(a, i);
+    out[i] = combine_scalar(a, i);
   }
 }
 
@@ -114,18 +156,85 @@ __global__ __launch_bounds__(256) void stage_adjoint_kernel(AdjointArgs<T> a, in
   }
 }
 
+template <typename T>
+__device__ __forceinline__ void target_scalar(const TargetArgs<T>& a, int64_t i) {
+  const T g = a.g[i];
+  const T t = a.dt * g;
+#pragma unroll
+  for (int j = 0; j < kMaxTargets; ++j) {
+    if (j >= a.n) break;
+    const int kind = a.kind[j];
+    const T v = (kind & DINO_TARGET_IDENTITY) ? g : a.c[j] * t;
+    if (kind & DINO_TARGET_RANGED) {
+      if (i < a.ib || i >= a.ie) continue;
+      T* o = a.out[j] + (i - a.go);
+      *o = (kind & DINO_TARGET_ACCUMULATE) ? *o + v : v;
+    } else {
+      T* o = a.out[j] + i;
+      *o = (kind & DINO_TARGET_ACCUMULATE) ? *o + v : v;
+    }
+  }
+}
+
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void targets_kernel(TargetArgs<T> a, int64_t n) {
+  typedef typename Vec<T>::type V;
+  constexpr int W = VEC ? Vec<T>::W : 1;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nv = n / W;
+  if (k < nv) {
+    if constexpr (VEC) {
+      const int64_t i0 = k * W;
+      const V g = reinterpret_cast<const V*>(a.g)[k];
+      const V t = a.dt * g;
+      const bool inside = i0 >= a.ib && i0 + W <= a.ie;
+      const bool touch = i0 + W > a.ib && i0 < a.ie;
+#pragma unroll
+      for (int j = 0; j < kMaxTargets; ++j) {
+        if (j >= a.n) break;
+        const int kind = a.kind[j];
+        const bool acc = (kind & DINO_TARGET_ACCUMULATE) != 0;
+        const V v = (kind & DINO_TARGET_IDENTITY) ? g : a.c[j] * t;
+        if (!(kind & DINO_TARGET_RANGED)) {
+          V* o = reinterpret_cast<V*>(a.out[j]) + k;
+          *o = acc ? *o + v : v;
+        } else if (inside) {
+          V* o = reinterpret_cast<V*>(a.out[j]) + (i0 - a.go) / W;
+          *o = acc ? *o + v : v;
+        } else if (touch) {
+#pragma unroll
+          for (int e = 0; e < W; ++e) {
+            const int64_t i = i0 + e;
+            if (i >= a.ib && i < a.ie) {
+              T* o = a.out[j] + (i - a.go);
+              *o = acc ? *o + v[e] : v[e];
+            }
+          }
+        }
+      }
+    } else {
+      target_scalar(a, k);
+    }
+  } else if (VEC && k - nv < n - nv * W) {
+    target_scalar(a, nv * W + (k - nv));
+  }
+}
+
 inline bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
 
 template <typename T>
 int launch_stage(const void* y0, int n_ex, const void* const* f, const double* a, int n_im,
                  const void* const* g, const double* b, double dt, void* out, int64_t n,
-                 hipStream_t s) {
+                 int64_t ib, int64_t ie, int64_t go, hipStream_t s) {
   StageArgs<T> args{};
   args.y0 = static_cast<const T*>(y0);
   args.n[0] = n_ex;
   args.n[1] = n_im;
   args.dt = (T)dt;
-  bool vec = aligned16(y0) && aligned16(out);
+  args.ib = ib;
+  args.ie = ie;
+  args.go = go;
+  bool vec = aligned16(y0) && aligned16(out) && go % Vec<T>::W == 0;
   for (int j = 0; j < n_ex; ++j) {
     args.x[0][j] = static_cast<const T*>(f[j]);
     args.c[0][j] = (T)a[j];
@@ -178,6 +287,42 @@ int launch_adjoint(const void* gout, int n_ex, const double* a, int n_im, const 
   return DINO_OK;
 }
 
+template <typename T>
+int launch_targets(const void* gbar, int nt, void* const* outp, const double* coef,
+                   const int* kind, double dt, int64_t n, int64_t ib, int64_t ie, int64_t go,
+                   hipStream_t s) {
+  TargetArgs<T> args{};
+  args.g = static_cast<const T*>(gbar);
+  args.n = nt;
+  args.dt = (T)dt;
+  args.ib = ib;
+  args.ie = ie;
+  args.go = go;
+  bool vec = aligned16(gbar) && go % Vec<T>::W == 0;
+  for (int j = 0; j < nt; ++j) {
+    args.out[j] = static_cast<T*>(outp[j]);
+    args.c[j] = (T)coef[j];
+    args.kind[j] = kind[j];
+    vec = vec && aligned16(outp[j]);
+  }
+  const int W = vec ? Vec<T>::W : 1;
+  const int64_t threads = n / W + (n % W);
+  const unsigned blocks = (unsigned)((threads + 255) / 256);
+  if (vec)
+    targets_kernel<T, true><<<blocks, 256, 0, s>>>(args, n);
+  else
+    targets_kernel<T, false><<<blocks, 256, 0, s>>>(args, n);
+  DINO_LAUNCH_CHECK("imex stage targets kernel");
+  return DINO_OK;
+}
+
+int check_range(const char* name, int64_t n, int64_t ib, int64_t ie, int64_t go) {
+  DINO_REQUIRE(0 <= go && go <= ib && ib <= ie && ie <= n, DINO_EINVAL,
+               "%s: need 0 <= g_origin <= im_begin <= im_end <= n (got %lld %lld %lld %lld)",
+               name, (long long)go, (long long)ib, (long long)ie, (long long)n);
+  return DINO_OK;
+}
+
 int check_args(const char* name, int n_ex, int n_im, const double* a, const double* b,
                int64_t n, int dtype) {
   DINO_REQUIRE(n_ex >= 0 && n_ex <= kMaxTerms && n_im >= 0 && n_im <= kMaxTerms, DINO_EINVAL,
@@ -206,8 +351,61 @@ extern "C" int dino_imex_stage(const void* y0, int n_ex, const void* const* f, c
   for (int j = 0; j < n_im; ++j) DINO_REQUIRE(g[j], DINO_EINVAL, "dino_imex_stage: NULL g");
   if (n == 0) return DINO_OK;
   if (dtype == DINO_F64)
-    return launch_stage<double>(y0, n_ex, f, a, n_im, g, b, dt, out, n, as_stream(stream));
-  return launch_stage<float>(y0, n_ex, f, a, n_im, g, b, dt, out, n, as_stream(stream));
+    return launch_stage<double>(y0, n_ex, f, a, n_im, g, b, dt, out, n, 0, n, 0,
+                                as_stream(stream));
+  return launch_stage<float>(y0, n_ex, f, a, n_im, g, b, dt, out, n, 0, n, 0,
+                             as_stream(stream));
+}
+
+extern "C" int dino_imex_stage_ranged(const void* y0, int n_ex, const void* const* f,
+                                      const double* a, int n_im, const void* const* g,
+                                      const double* b, double dt, void* out, int64_t n,
+                                      int64_t im_begin, int64_t im_end, int64_t g_origin,
+                                      int dtype, void* stream) {
+  const char* name = "dino_imex_stage_ranged";
+  int rc = check_args(name, n_ex, n_im, a, b, n, dtype);
+  if (rc != DINO_OK) return rc;
+  rc = check_range(name, n, im_begin, im_end, g_origin);
+  if (rc != DINO_OK) return rc;
+  DINO_REQUIRE(y0 && out && (n_ex == 0 || f) && (n_im == 0 || g), DINO_EINVAL,
+               "%s: NULL argument", name);
+  for (int j = 0; j < n_ex; ++j) DINO_REQUIRE(f[j], DINO_EINVAL, "%s: NULL f", name);
+  for (int j = 0; j < n_im; ++j) DINO_REQUIRE(g[j], DINO_EINVAL, "%s: NULL g", name);
+  if (n == 0) return DINO_OK;
+  if (dtype == DINO_F64)
+    return launch_stage<double>(y0, n_ex, f, a, n_im, g, b, dt, out, n, im_begin, im_end,
+                                g_origin, as_stream(stream));
+  return launch_stage<float>(y0, n_ex, f, a, n_im, g, b, dt, out, n, im_begin, im_end,
+                             g_origin, as_stream(stream));
+}
+
+extern "C" int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
+                                               void* const* out, const double* coef,
+                                               const int* kind, double dt, int64_t n,
+                                               int64_t im_begin, int64_t im_end,
+                                               int64_t g_origin, int dtype, void* stream) {
+  const char* name = "dino_imex_stage_adjoint_targets";
+  DINO_REQUIRE(n_targets >= 0 && n_targets <= kMaxTargets, DINO_EINVAL,
+               "%s: at most %d targets (got %d)", name, kMaxTargets, n_targets);
+  DINO_REQUIRE(gbar && (n_targets == 0 || (out && coef && kind)), DINO_EINVAL,
+               "%s: NULL argument", name);
+  DINO_REQUIRE(n >= 0, DINO_EINVAL, "%s: negative size", name);
+  DINO_REQUIRE(dtype == DINO_F32 || dtype == DINO_F64, DINO_EINVAL, "%s: invalid dtype %d",
+               name, dtype);
+  int rc = check_range(name, n, im_begin, im_end, g_origin);
+  if (rc != DINO_OK) return rc;
+  for (int j = 0; j < n_targets; ++j) {
+    DINO_REQUIRE(out[j], DINO_EINVAL, "%s: NULL target %d", name, j);
+    DINO_REQUIRE((kind[j] & ~(DINO_TARGET_ACCUMULATE | DINO_TARGET_IDENTITY |
+                              DINO_TARGET_RANGED)) == 0,
+                 DINO_EINVAL, "%s: invalid kind %d", name, kind[j]);
+  }
+  if (n == 0 || n_targets == 0) return DINO_OK;
+  if (dtype == DINO_F64)
+    return launch_targets<double>(gbar, n_targets, out, coef, kind, dt, n, im_begin, im_end,
+                                  g_origin, as_stream(stream));
+  return launch_targets<float>(gbar, n_targets, out, coef, kind, dt, n, im_begin, im_end,
+                               g_origin, as_stream(stream));
 }
 
 extern "C" int dino_imex_stage_adjoint(const void* out_cot, int n_ex, const double* a,

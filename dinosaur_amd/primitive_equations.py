@@ -568,6 +568,80 @@ class _ImplicitFlat(torch.autograd.Function):
     return gin, None, None, None, None, None
 
 
+class _FlatImexOps:
+  """Flat-buffer operations of one state layout for the fused IMEX step
+  (time_integration._fused_imex_step): explicit terms (differentiable),
+  implicit terms stored only over the divergence / temperature / ln ps range
+  (vorticity and tracers have none, primitive_equations.py:1230-1249), their
+  accumulating adjoint, and the implicit solve and its adjoint in place.
+
+  A compact implicit buffer holds flat elements [go, ie) of a state, where
+  [ib, ie) is the divergence..ln ps range and go = ib rounded down to a
+  multiple of 4, so that 16-byte vectors of full and compact buffers line up.
+  """
+
+  def __init__(self, eq, layout, members):
+    self.eq = eq
+    self.layout = layout
+    lead, K, names = layout
+    self.members = members
+    self.K = K
+    self.names = names
+    R, L = eq.coords.horizontal.modal_shape
+    self.RL = R * L
+    E = members
+    self.ib = E * K * self.RL
+    self.ie = (3 * E * K + E) * self.RL
+    self.go = self.ib - self.ib % 4
+    self.n = (3 * E * K + E + E * K * len(names)) * self.RL
+    self.op = eq._implicit
+
+  def state(self, buf):
+    return State.from_flat(buf, *self.layout)
+
+  def explicit(self, buf):
+    out = self.eq.explicit_terms(self.state(buf))
+    fb = out.flat_buffer()
+    return fb if fb is not None else to_flat(out).flat_buffer()
+
+  def _views(self, full):
+    E, K, RL = self.members, self.K, self.RL
+    d = full[E * K:2 * E * K]
+    t = full[2 * E * K:3 * E * K]
+    l = full[3 * E * K:3 * E * K + E]
+    return d, t, l
+
+  def _compact_views(self, comp):
+    E, K, RL = self.members, self.K, self.RL
+    o = self.ib - self.go
+    d = comp[o:o + E * K * RL]
+    t = comp[o + E * K * RL:o + 2 * E * K * RL]
+    l = comp[o + 2 * E * K * RL:o + (2 * E * K + E) * RL]
+    return d, t, l
+
+  def new_compact(self, like):
+    return torch.empty(self.ie - self.go, dtype=like.dtype, device=like.device)
+
+  def terms(self, buf):
+    comp = self.new_compact(buf)
+    self.op.apply_into('dino_implicit_terms', 0.0, self.members, *self._views(buf),
+                       *self._compact_views(comp))
+    return comp
+
+  def terms_adjoint_acc(self, gcomp, ybar):
+    """ybar[divergence..ln ps] += (implicit terms)^T gcomp."""
+    self.op.apply_into('dino_implicit_terms_adjoint_accumulate', 0.0, self.members,
+                       *self._compact_views(gcomp), *self._views(ybar))
+
+  def solve_(self, buf, eta):
+    v = self._views(buf)
+    self.op.apply_into('dino_implicit_solve', float(eta), self.members, *v, *v)
+
+  def solve_adjoint_(self, buf, eta):
+    v = self._views(buf)
+    self.op.apply_into('dino_implicit_solve_adjoint', float(eta), self.members, *v, *v)
+
+
 # ------------------------------------------------------------------ equations
 
 
@@ -925,6 +999,21 @@ class PrimitiveEquationsSigma:
         tracers={k: torch.zeros_like(v) for k, v in state.tracers.items()},
         sim_time=None if state.sim_time is None else 0.0,
     )
+
+  def _flat_imex_ops(self, state):
+    """Flat-buffer hooks for the fused IMEX step, or None when `state` is not
+    flat or a subclass replaces the implicit methods."""
+    cls = type(self)
+    if (cls.implicit_terms is not PrimitiveEquationsSigma.implicit_terms or
+        cls.implicit_inverse is not PrimitiveEquationsSigma.implicit_inverse):
+      return None
+    buf = state.flat_buffer()
+    if buf is None or not buf.is_cuda:
+      return None
+    validate_state_shape(state, self.coords)
+    lead, K, names = state._flat[1:]
+    members = int(np.prod(lead)) if lead else 1
+    return _FlatImexOps(self, (lead, K, names), members)
 
   def implicit_inverse(self, state: State, step_size: float) -> State:
     """(1 - step_size * implicit_terms)^{-1} (primitive_equations.py:1251-1458)."""

@@ -200,6 +200,178 @@ def _stage(y0, dt, ex, im):
   return out
 
 
+def _ptrs(ts):
+  return (ctypes.c_void_p * max(len(ts), 1))(*[t.data_ptr() for t in ts])
+
+
+def _dbls(cs):
+  return (ctypes.c_double * max(len(cs), 1))(*[float(c) for c in cs])
+
+
+def _code(t):
+  from . import _lib  # pylint: disable=g-import-not-at-top
+  return _lib.DINO_F64 if t.dtype == torch.float64 else _lib.DINO_F32
+
+
+def _fused_stage(ops, y0, dt, ex, im):
+  """y0 + dt sum(a f) + dt sum(b g) with compact implicit terms g (one pass)."""
+  from . import _lib  # pylint: disable=g-import-not-at-top
+  out = torch.empty_like(y0)
+  fs, gs = [x for _, x in ex], [x for _, x in im]
+  stream = ctypes.c_void_p(torch.cuda.current_stream(y0.device).cuda_stream)
+  rc = _lib.lib().dino_imex_stage_ranged(
+      ctypes.c_void_p(y0.data_ptr()), len(fs), _ptrs(fs), _dbls([a for a, _ in ex]), len(gs),
+      _ptrs(gs), _dbls([b for b, _ in im]), float(dt), ctypes.c_void_p(out.data_ptr()),
+      y0.numel(), ops.ib, ops.ie, ops.go, _code(y0), stream)
+  _lib.check(rc, 'dino_imex_stage_ranged')
+  return out
+
+
+def _fused_targets(ops, gbar, dt, targets):
+  """One pass over gbar writing or accumulating every (buffer, coef, kind)."""
+  from . import _lib  # pylint: disable=g-import-not-at-top
+  if not targets:
+    return
+  stream = ctypes.c_void_p(torch.cuda.current_stream(gbar.device).cuda_stream)
+  kinds = (ctypes.c_int * len(targets))(*[k for _, _, k in targets])
+  rc = _lib.lib().dino_imex_stage_adjoint_targets(
+      ctypes.c_void_p(gbar.data_ptr()), len(targets), _ptrs([b for b, _, _ in targets]),
+      _dbls([c for _, c, _ in targets]), kinds, float(dt), gbar.numel(), ops.ib, ops.ie,
+      ops.go, _code(gbar), stream)
+  _lib.check(rc, 'dino_imex_stage_adjoint_targets')
+
+
+def _fused_forward(ops, tab, dt, y0, keep):
+  """The stages of one IMEX step on flat buffers.  keep=True records each
+  explicit evaluation's autograd graph (leaf input, output) for the reverse
+  sweep."""
+  a_ex, a_im, b_ex, b_im = tab.a_ex, tab.a_im, tab.b_ex, tab.b_im
+  n = len(b_ex)
+  f, g, leaves = [None] * n, [None] * n, [None] * n
+
+  def explicit(i, y):
+    if keep:
+      leaves[i] = y.detach().requires_grad_(True)
+      with torch.enable_grad():
+        f[i] = ops.explicit(leaves[i])
+    else:
+      f[i] = ops.explicit(y)
+
+  explicit(0, y0)
+  g[0] = ops.terms(y0)
+  for i in range(1, n):
+    ex = [(a_ex[i - 1][j], f[j]) for j in range(i) if a_ex[i - 1][j]]
+    im = [(a_im[i - 1][j], g[j]) for j in range(i) if a_im[i - 1][j]]
+    y = _fused_stage(ops, y0, dt, ex, im)
+    ops.solve_(y, dt * a_im[i - 1][i])
+    if any(a_ex[j][i] for j in range(i, n - 1)) or b_ex[i]:
+      explicit(i, y)
+    if any(a_im[j][i] for j in range(i, n - 1)) or b_im[i]:
+      g[i] = ops.terms(y)
+  ex = [(b_ex[j], f[j]) for j in range(n) if b_ex[j]]
+  im = [(b_im[j], g[j]) for j in range(n) if b_im[j]]
+  return _fused_stage(ops, y0, dt, ex, im), f, g, leaves
+
+
+class _FusedImexStep(torch.autograd.Function):
+  """One IMEX step with a hand-written reverse sweep.
+
+  The forward keeps, for each explicit evaluation, its autograd graph; the
+  backward walks the stages in reverse: explicit adjoints through those
+  graphs, implicit-term adjoints accumulated in place, the solve adjoint in
+  place, and every stage combination's cotangents written or accumulated
+  in one pass (dino_imex_stage_adjoint_targets).  The result equals
+  autograd through the same stages up to FP rounding of the sums.
+  """
+
+  @staticmethod
+  def forward(ctx, ops, tab, dt, y0):
+    out, f, g, leaves = _fused_forward(ops, tab, dt, y0, keep=True)
+    ctx.ops, ctx.tab, ctx.dt = ops, tab, dt
+    ctx.f, ctx.leaves = f, leaves
+    ctx.has_g = [x is not None for x in g]
+    return out
+
+  @staticmethod
+  def backward(ctx, obar):
+    from . import _lib  # pylint: disable=g-import-not-at-top
+    ops, tab, dt, f, leaves = ctx.ops, ctx.tab, ctx.dt, ctx.f, ctx.leaves
+    a_ex, a_im, b_ex, b_im = tab.a_ex, tab.a_im, tab.b_ex, tab.b_im
+    ACC, IDN, RNG = _lib.TARGET_ACCUMULATE, _lib.TARGET_IDENTITY, _lib.TARGET_RANGED
+    obar = obar.contiguous()
+    n = len(b_ex)
+    fbar, gbar = [None] * n, [None] * n
+
+    def targets_for(coef_ex, coef_im, first_y0):
+      out = [(ybar0, 1.0, IDN if first_y0 else IDN | ACC)]
+      for j, c in coef_ex:
+        if fbar[j] is None:
+          fbar[j] = torch.empty_like(obar)
+          out.append((fbar[j], c, 0))
+        else:
+          out.append((fbar[j], c, ACC))
+      for j, c in coef_im:
+        if gbar[j] is None:
+          gbar[j] = ops.new_compact(obar)
+          out.append((gbar[j], c, RNG))
+        else:
+          out.append((gbar[j], c, RNG | ACC))
+      return out
+
+    ybar0 = torch.empty_like(obar)
+    _fused_targets(ops, obar, dt, targets_for(
+        [(j, b_ex[j]) for j in range(n) if b_ex[j] and f[j] is not None],
+        [(j, b_im[j]) for j in range(n) if b_im[j] and ctx.has_g[j]], True))
+    for i in range(n - 1, 0, -1):
+      ybar = None
+      if f[i] is not None and fbar[i] is not None:
+        (ybar,) = torch.autograd.grad(f[i], leaves[i], fbar[i])
+      if ctx.has_g[i] and gbar[i] is not None:
+        if ybar is None:
+          ybar = torch.zeros_like(obar)
+        ops.terms_adjoint_acc(gbar[i], ybar)
+      f[i] = leaves[i] = fbar[i] = gbar[i] = None
+      if ybar is None:
+        continue
+      ybar = ybar.contiguous()
+      ops.solve_adjoint_(ybar, dt * a_im[i - 1][i])
+      _fused_targets(ops, ybar, dt, targets_for(
+          [(j, a_ex[i - 1][j]) for j in range(i) if a_ex[i - 1][j] and f[j] is not None],
+          [(j, a_im[i - 1][j]) for j in range(i) if a_im[i - 1][j] and ctx.has_g[j]], False))
+    if f[0] is not None and fbar[0] is not None:
+      (gy0,) = torch.autograd.grad(f[0], leaves[0], fbar[0])
+      ybar0.add_(gy0)
+    if ctx.has_g[0] and gbar[0] is not None:
+      ops.terms_adjoint_acc(gbar[0], ybar0)
+    ctx.f = ctx.leaves = None
+    return None, None, None, ybar0
+
+
+def _fused_imex_step(ops, tab, dt, y0):
+  """One IMEX step on a flat state through the fused kernels, or None when
+  the tableau does not fit them (at most 4 terms per group)."""
+  n = len(tab.b_ex)
+  rows = [tab.a_ex[i] for i in range(n - 1)] + [tab.b_ex]
+  rows_im = [tab.a_im[i] for i in range(n - 1)] + [tab.b_im]
+  if any(sum(1 for c in r if c) > 4 for r in rows + rows_im):
+    return None
+  buf = y0.flat_buffer()
+  if torch.is_grad_enabled() and buf.requires_grad:
+    out = _FusedImexStep.apply(ops, tab, dt, buf)
+  else:
+    with torch.no_grad():
+      out = _fused_forward(ops, tab, dt, buf, keep=False)[0]
+  t0 = y0.sim_time
+  st = None
+  if t0 is not None:
+    s = None
+    for b in tab.b_ex:
+      if b:
+        s = b if s is None else s + b
+    st = t0 if s is None else t0 + dt * s
+  return type(y0).from_flat(out, *y0._flat[1:], sim_time=st)
+
+
 def imex_runge_kutta(tableau: ImExButcherTableau, equation, time_step: float):
   """IMEX Runge-Kutta step (time_integration.py:333-370)."""
   dt = time_step
@@ -207,11 +379,18 @@ def imex_runge_kutta(tableau: ImExButcherTableau, equation, time_step: float):
   a_ex, a_im, b_ex, b_im = tableau.a_ex, tableau.a_im, tableau.b_ex, tableau.b_im
   n = len(b_ex)
 
+  hook = getattr(equation, '_flat_imex_ops', None)
+
   def step_fn(y0):
     # one flat buffer per state: every stage combination is a single pass
     as_flat = getattr(y0, 'as_flat', None)
     if as_flat is not None:
       y0 = as_flat()
+    ops = hook(y0) if (hook is not None and as_flat is not None) else None
+    if ops is not None:
+      out = _fused_imex_step(ops, tableau, dt, y0)
+      if out is not None:
+        return out
     f = [None] * n
     g = [None] * n
     f[0] = F(y0)
@@ -399,11 +578,42 @@ def checkpointed_rollout(step_fn, x, steps: int, checkpoint_every: int):
   for _ in range(steps // checkpoint_every):
     fn = functools.partial(segment, rebuild)
     if torch.is_grad_enabled() and any(t.requires_grad for t in leaves):
-      leaves = list(torch.utils.checkpoint.checkpoint(fn, *leaves, use_reentrant=False))
+      leaves = list(_CheckpointSegment.apply(fn, *leaves))
     else:
       leaves = list(fn(*leaves))
     rebuild = structure['rebuild']
   return rebuild(leaves)
+
+
+class _CheckpointSegment(torch.autograd.Function):
+  """Runs a segment of steps without recording anything; the backward pass
+  re-runs it from the saved input with gradients enabled and differentiates
+  that recomputation.  Steps that record their own graphs when gradients are
+  enabled (the fused IMEX step) therefore record them once per segment, in
+  the backward pass, and never during the first forward."""
+
+  @staticmethod
+  def forward(ctx, fn, *leaves):
+    ctx.fn = fn
+    ctx.save_for_backward(*leaves)
+    with torch.no_grad():
+      outs = fn(*leaves)
+    # outputs must not alias inputs (autograd would see them as views)
+    ids = {t.data_ptr() for t in leaves}
+    return tuple(o.clone() if o.data_ptr() in ids else o for o in outs)
+
+  @staticmethod
+  def backward(ctx, *gouts):
+    saved = ctx.saved_tensors
+    inputs = [t.detach().requires_grad_(t.requires_grad) for t in saved]
+    with torch.enable_grad():
+      outs = ctx.fn(*inputs)
+    pairs = [(o, g) for o, g in zip(outs, gouts) if g is not None and o.requires_grad]
+    want = [t for t in inputs if t.requires_grad]
+    grads = torch.autograd.grad([o for o, _ in pairs], want, [g for _, g in pairs],
+                                allow_unused=True) if pairs and want else [None] * len(want)
+    it = iter(grads)
+    return (None,) + tuple(next(it) if t.requires_grad else None for t in inputs)
 
 
 def nested_checkpoint_scan(f, init, xs, length=None, *, nested_lengths, scan_fn=None,

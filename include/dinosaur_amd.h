@@ -175,7 +175,9 @@ int dino_implicit_create(int K, int L, int R, const double* G, const double* H,
 int dino_implicit_destroy(dino_implicit* f);
 
 /* Reads div/temp (members, K, R, L) and lnps (members, 1, R, L) and writes
- * the three outputs, which must not alias the inputs. */
+ * the three outputs.  Outputs either do not overlap the inputs or equal them
+ * (in place: div_out == div, temp_out == temp, lnps_out == lnps); the same
+ * holds for dino_implicit_terms and both adjoints. */
 int dino_implicit_solve(const dino_implicit* f, const void* div,
                         const void* temp, const void* lnps, void* div_out,
                         void* temp_out, void* lnps_out, int64_t members,
@@ -205,6 +207,17 @@ int dino_implicit_terms_adjoint(const dino_implicit* f, const void* div_cot,
                                 const void* temp_cot, const void* lnps_cot,
                                 void* div_out, void* temp_out, void* lnps_out,
                                 int64_t members, int dtype, void* stream);
+/* As dino_implicit_terms_adjoint, but adds the cotangents to what the output
+ * buffers hold (out += G^T cot): the accumulation jax.vjp performs when a
+ * state feeds both explicit_terms and implicit_terms
+ * (dinosaur/time_integration.py:361-364), fused into the kernel. */
+int dino_implicit_terms_adjoint_accumulate(const dino_implicit* f,
+                                           const void* div_cot,
+                                           const void* temp_cot,
+                                           const void* lnps_cot, void* div_out,
+                                           void* temp_out, void* lnps_out,
+                                           int64_t members, int dtype,
+                                           void* stream);
 
 /* IMEX Runge-Kutta stage combination on flat state buffers of n elements:
  *   out = y0 + dt * (a[0] f[0] + ... + a[n_ex-1] f[n_ex-1])
@@ -227,6 +240,42 @@ int dino_imex_stage_adjoint(const void* out_cot, int n_ex, const double* a,
                             int n_im, const double* b, double dt,
                             void* const* f_cot, void* const* g_cot, int64_t n,
                             int dtype, void* stream);
+
+/* dino_imex_stage with implicit terms stored only where they can be nonzero:
+ * element i of every g[k] is g[k][i - g_origin] for im_begin <= i < im_end,
+ * and the implicit group adds nothing outside that range (the reference
+ * adds dt * b * 0 there: the same value).  For the primitive equations the
+ * range is the divergence, temperature and ln ps part of the flat state:
+ * vorticity and tracers have no implicit tendency
+ * (dinosaur/primitive_equations.py:1230-1249).  out may alias y0. */
+int dino_imex_stage_ranged(const void* y0, int n_ex, const void* const* f,
+                           const double* a, int n_im, const void* const* g,
+                           const double* b, double dt, void* out, int64_t n,
+                           int64_t im_begin, int64_t im_end, int64_t g_origin,
+                           int dtype, void* stream);
+
+/* Cotangent targets of one stage combination, written in one pass over the
+ * output cotangent gbar: for each target t,
+ *   v = gbar                       (kind has DINO_TARGET_IDENTITY: y0),
+ *   v = coef[t] * (dt * gbar)      (otherwise: an f or g term),
+ *   out[t] = out[t] + v            (kind has DINO_TARGET_ACCUMULATE),
+ *   out[t] = v                     (otherwise);
+ * DINO_TARGET_RANGED targets are ranged implicit terms (indexing as in
+ * dino_imex_stage_ranged; nothing is written outside the range).  This is
+ * the reverse sweep of imex_runge_kutta (time_integration.py:351-370) with
+ * the accumulation of each term's cotangent over the stages that use it
+ * fused into the pass.  At most 9 targets; out buffers must not overlap
+ * gbar or each other. */
+enum {
+  DINO_TARGET_ACCUMULATE = 1,
+  DINO_TARGET_IDENTITY = 2,
+  DINO_TARGET_RANGED = 4,
+};
+int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
+                                    void* const* out, const double* coef,
+                                    const int* kind, double dt, int64_t n,
+                                    int64_t im_begin, int64_t im_end,
+                                    int64_t g_origin, int dtype, void* stream);
 
 /* Fused nodal nonlinear terms of PrimitiveEquationsSigma.explicit_terms.
  * Replaces compute_diagnostic_state_sigma and the nodal parts of

@@ -113,51 +113,63 @@ struct MP {
   double a, g;
 };
 
+// Stencil of one thread's (r, l): the bidiagonal-in-l operators read the
+// neighbours l - 1 and l + 1.  Out-of-range neighbours are loaded from a
+// clamped index and discarded by a select, so no load sits behind a branch
+// (a guarded load would wait on its own); the per-(r, l) coefficients are
+// read once, since they are the same for every field and level.  Each
+// operator adds its two terms in the same order as a sequential
+// v = 0; v += first; v += second, with zero for a missing term.
 template <typename T>
 struct Stencil {
-  const T* c;
-  MOff o;
-  int R, L, r, l, pr;
+  int L, row, l, lm, lp, prow;
+  bool hm, hp;
   T jr;
-  __device__ Stencil(const T* c_, MOff o_, int R_, int L_, int r_, int l_)
-      : c(c_), o(o_), R(R_), L(L_), r(r_), l(l_) {
+  T up_p, dn_m, up2_p, dn2_m;  // forward operators: coefficients at l + 1, l - 1
+  T up_l, dn_l, up2_l, dn2_l;  // transposes: coefficients at l
+  T il_p, il_m;                // inverse Laplacian at l + 1, l - 1
+  __device__ Stencil(const T* c, MOff o, int R_, int L_, int r, int l_) : L(L_), l(l_) {
+    (void)R_;
+    row = r * L;
+    lm = l > 0 ? l - 1 : 0;
+    lp = l + 1 < L ? l + 1 : L - 1;
+    hm = l >= 1;
+    hp = l + 1 < L;
     jr = c[o.jr + r];
-    pr = (int)c[o.pr + r];
+    prow = (int)c[o.pr + r] * L;
+    up_p = c[o.up + row + lp];
+    dn_m = c[o.dn + row + lm];
+    up2_p = c[o.up2 + row + lp];
+    dn2_m = c[o.dn2 + row + lm];
+    up_l = c[o.up + row + l];
+    dn_l = c[o.dn + row + l];
+    up2_l = c[o.up2 + row + l];
+    dn2_l = c[o.dn2 + row + l];
+    il_p = c[o.il + lp];
+    il_m = c[o.il + lm];
   }
-  // x: field base (R*L), value at (rr, ll) with zero outside
-  __device__ T at(const T* x, int ll) const { return (ll >= 0 && ll < L) ? x[r * L + ll] : (T)0; }
-  __device__ T dlon(const T* x) const { return jr * x[pr * L + l]; }
+  __device__ T dlon(const T* x) const { return jr * x[prow + l]; }
   __device__ T cldl(const T* x) const {
-    T v = 0;
-    if (l + 1 < L) v += c[o.up + r * L + l + 1] * x[r * L + l + 1];
-    if (l >= 1) v += c[o.dn + r * L + l - 1] * x[r * L + l - 1];
-    return v;
+    const T a = up_p * x[row + lp], b = dn_m * x[row + lm];
+    return (hp ? a : (T)0) + (hm ? b : (T)0);
   }
   __device__ T sdlc(const T* x) const {
-    T v = 0;
-    if (l + 1 < L) v += c[o.up2 + r * L + l + 1] * x[r * L + l + 1];
-    if (l >= 1) v += c[o.dn2 + r * L + l - 1] * x[r * L + l - 1];
-    return v;
+    const T a = up2_p * x[row + lp], b = dn2_m * x[row + lm];
+    return (hp ? a : (T)0) + (hm ? b : (T)0);
   }
   // transposes: cldl^T(y)[l] = UP[l] y[l-1] + DN[l] y[l+1]
   __device__ T cldl_t(const T* y) const {
-    T v = 0;
-    if (l >= 1) v += c[o.up + r * L + l] * y[r * L + l - 1];
-    if (l + 1 < L) v += c[o.dn + r * L + l] * y[r * L + l + 1];
-    return v;
+    const T a = up_l * y[row + lm], b = dn_l * y[row + lp];
+    return (hm ? a : (T)0) + (hp ? b : (T)0);
   }
   __device__ T sdlc_t(const T* y) const {
-    T v = 0;
-    if (l >= 1) v += c[o.up2 + r * L + l] * y[r * L + l - 1];
-    if (l + 1 < L) v += c[o.dn2 + r * L + l] * y[r * L + l + 1];
-    return v;
+    const T a = up2_l * y[row + lm], b = dn2_l * y[row + lp];
+    return (hm ? a : (T)0) + (hp ? b : (T)0);
   }
   // cldl of (il * x): the inverse Laplacian applied at the stencil points
   __device__ T cldl_il(const T* x) const {
-    T v = 0;
-    if (l + 1 < L) v += c[o.up + r * L + l + 1] * c[o.il + l + 1] * x[r * L + l + 1];
-    if (l >= 1) v += c[o.dn + r * L + l - 1] * c[o.il + l - 1] * x[r * L + l - 1];
-    return v;
+    const T a = up_p * il_p * x[row + lp], b = dn_m * il_m * x[row + lm];
+    return (hp ? a : (T)0) + (hm ? b : (T)0);
   }
 };
 
@@ -307,35 +319,35 @@ __global__ __launch_bounds__(256) void epilogue_adjoint_kernel(const T* __restri
   auto W = [&](T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
   // cotangents of the clipped outputs: y' = mask * ybar (mask is diagonal),
   // read through a masked view so stencil neighbours are masked too
-  auto MB = [&](const T* y, int ll) -> T { return y[r * p.L + ll] * cst[o.mask + ll]; };
-  auto dlon_m = [&](const T* y) -> T { return S.jr * y[S.pr * p.L + l] * cst[o.mask + l]; };
-  auto cldl_t_m = [&](const T* y, int up, int dn) -> T {
-    T v = 0;
-    if (l >= 1) v += cst[up + r * p.L + l] * MB(y, l - 1);
-    if (l + 1 < p.L) v += cst[dn + r * p.L + l] * MB(y, l + 1);
-    return v;
+  const T mk_l = cst[o.mask + l], mk_m = cst[o.mask + S.lm], mk_p = cst[o.mask + S.lp];
+  auto MB = [&](const T* y) -> T { return y[S.row + l] * mk_l; };
+  auto dlon_m = [&](const T* y) -> T { return S.jr * y[S.prow + l] * mk_l; };
+  // sdlc^T of the masked cotangent (the neighbours' masks at l -+ 1)
+  auto sdlc_t_m = [&](const T* y) -> T {
+    const T a = S.up2_l * (y[S.row + S.lm] * mk_m), b = S.dn2_l * (y[S.row + S.lp] * mk_p);
+    return (S.hm ? a : (T)0) + (S.hp ? b : (T)0);
   };
   const bool moist = p.hum >= 0;
   for (int k = 0; k < K; ++k) {
     const T* zb = F(gst, sg.z, K, k);
     const T* db = F(gst, sg.d, K, k);
     const T* tb = F(gst, sg.t, K, k);
-    const T zm = MB(zb, l), dm = MB(db, l), tm = MB(tb, l);
+    const T zm = MB(zb), dm = MB(db), tm = MB(tb);
     // vort = (sdlc(cu) - dlon(cv)) / a ; div = -(dlon(cu) + sdlc(cv)) / a - ev ke ...
-    T cub = (cldl_t_m(zb, o.up2, o.dn2) + dlon_m(db)) * inva;
-    T cvb = (dlon_m(zb) - cldl_t_m(db, o.up2, o.dn2)) * inva;
+    T cub = (sdlc_t_m(zb) + dlon_m(db)) * inva;
+    T cvb = (dlon_m(zb) - sdlc_t_m(db)) * inva;
     *(W(gin, eg.cu, K, k) + pos) = cub;
     *(W(gin, eg.cv, K, k) + pos) = cvb;
     *(W(gin, eg.ke, K, k) + pos) = -ev * dm;
     *(W(gin, eg.te, K, k) + pos) = tm;
     *(W(gin, eg.ut, K, k) + pos) = dlon_m(tb) * inva;
-    *(W(gin, eg.vt, K, k) + pos) = -cldl_t_m(tb, o.up2, o.dn2) * inva;
+    *(W(gin, eg.vt, K, k) + pos) = -sdlc_t_m(tb) * inva;
     for (int i = 0; i < MAXTR; ++i) {
       if (i >= p.nt) break;
       const T* xb = F(gst, sg.tr[i], K, k);
-      *(W(gin, eg.trn[i], K, k) + pos) = MB(xb, l);
+      *(W(gin, eg.trn[i], K, k) + pos) = MB(xb);
       *(W(gin, eg.tu[i], K, k) + pos) = dlon_m(xb) * inva;
-      *(W(gin, eg.tv[i], K, k) + pos) = -cldl_t_m(xb, o.up2, o.dn2) * inva;
+      *(W(gin, eg.tv[i], K, k) + pos) = -sdlc_t_m(xb) * inva;
     }
     if (moist) {
       *(W(gin, eg.curl, K, k) + pos) = zm;
@@ -343,7 +355,7 @@ __global__ __launch_bounds__(256) void epilogue_adjoint_kernel(const T* __restri
       *(W(gin, eg.dc, K, k) + pos) = -dm;
     }
   }
-  *(W(gin, eg.lsp, 1, 0) + pos) = MB(F(gst, sg.lnps, 1, 0), l);
+  *(W(gin, eg.lsp, 1, 0) + pos) = MB(F(gst, sg.lnps, 1, 0));
 }
 
 template <typename T>

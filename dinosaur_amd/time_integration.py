@@ -654,3 +654,87 @@ def nested_checkpoint_scan(f, init, xs, length=None, *, nested_lengths, scan_fn=
     return carry, outs
 
   return run(init, 0, list(nested_lengths))
+
+
+# ---------------------------------------------------------------- digital filter initialization
+
+
+def _tree_map(fn, *trees):
+  """fn over matching array leaves of States (scalars such as sim_time are
+  taken from the first tree) or of plain tensors / arrays."""
+  first = trees[0]
+  if hasattr(first, '_ARRAYS'):
+    kw = {f: fn(*[getattr(t, f) for t in trees]) for f in first._ARRAYS}
+    kw['tracers'] = {k: fn(*[t.tracers[k] for t in trees]) for k in first.tracers}
+    kw['sim_time'] = first.sim_time
+    return type(first)(**kw)
+  if isinstance(first, (tuple, list)):
+    return type(first)(_tree_map(fn, *xs) for xs in zip(*trees))
+  if isinstance(first, dict):
+    return {k: _tree_map(fn, *[t[k] for t in trees]) for k in first}
+  return fn(*trees)
+
+
+class TimeReversedImExODE(ImplicitExplicitODE):
+  """dx/dt = -explicit_terms(x) - implicit_terms(x) (time_integration.py:118-140)."""
+
+  def __init__(self, forward_eq):
+    self.forward_eq = forward_eq
+
+  def explicit_terms(self, state):
+    return _tree_map(lambda x: -x, self.forward_eq.explicit_terms(state))
+
+  def implicit_terms(self, state):
+    return _tree_map(lambda x: -x, self.forward_eq.implicit_terms(state))
+
+  def implicit_inverse(self, state, step_size: float):
+    return self.forward_eq.implicit_inverse(state, -step_size)
+
+
+def accumulate_repeated(step_fn, weights, state, scan_fn=None):
+  """sum_i weights[i] * step_fn^(i+1)(state) (time_integration.py:645-659).
+  `scan_fn` is accepted for signature compatibility; the loop runs in Python."""
+  del scan_fn
+  averaged = _tree_map(lambda x: x * 0, state)
+  for w in weights:
+    state = step_fn(state)
+    averaged = _tree_map(lambda s, a, w=float(w): a + w * s, state, averaged)
+  return averaged
+
+
+def _dfi_lanczos_weights(time_span: float, cutoff_period: float, dt: float):
+  """Lanczos-windowed sinc weights for times 1..N (time_integration.py:662-669)."""
+  import numpy as np  # pylint: disable=g-import-not-at-top
+  N = round(time_span / (2 * dt))
+  n = np.arange(1, N + 1)
+  return np.sinc(n / (N + 1)) * np.sinc(n * time_span / (cutoff_period * N))
+
+
+def digital_filter_initialization(equation, ode_solver, filters, time_span: float,
+                                  cutoff_period: float, dt: float):
+  """State -> digitally filtered state: the Lanczos-weighted average of the
+  trajectory over [-time_span/2, time_span/2], integrated forward and, with
+  the time-reversed equation, backward from the same state (Lynch & Huang
+  1992; time_integration.py:672-720)."""
+
+  def f(state):
+    forward_step = step_with_filters(ode_solver(equation, dt), filters)
+    backward_step = step_with_filters(ode_solver(TimeReversedImExODE(equation), dt), filters)
+    weights = _dfi_lanczos_weights(time_span, cutoff_period, dt)
+    init_weight = 1.0
+    total_weight = init_weight + 2 * weights.sum()
+    init_weight /= total_weight
+    weights = weights / total_weight
+    init_term = _tree_map(lambda x: x * init_weight, state)
+    forward_term = accumulate_repeated(forward_step, weights, state)
+    backward_term = accumulate_repeated(backward_step, weights, state)
+    return _tree_map(lambda a, b, c: a + b + c, init_term, forward_term, backward_term)
+
+  return f
+
+
+def maybe_fix_sim_time_roundoff(state, dt: float):
+  """`state` with sim_time rounded to a whole number of steps (:723-730)."""
+  if hasattr(state, 'sim_time') and state.sim_time is not None:
+    state.sim_time = dt * round(state.sim_time / dt)
+  return state

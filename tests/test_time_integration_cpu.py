@@ -1,0 +1,52 @@
+"""Host-side checks of the generic time-integration utilities (they run on
+any torch tensors; the reference's own known answers,
+time_integration_test.py:303-335)."""
+import numpy as np
+import torch
+
+from dinosaur_amd import time_integration as ti
+
+
+def test_accumulate_repeated():
+  result = ti.accumulate_repeated(lambda x: 2 * x, torch.arange(4.0), torch.tensor(1.0))
+  assert result.item() == 0 * 2 + 1 * 4 + 2 * 8 + 3 * 16
+
+
+def test_dfi_lanczos_weights():
+  weights = ti._dfi_lanczos_weights(10, 10, 0.1)
+  assert weights.size == 50
+  assert (np.diff(weights) < 0).all()
+  assert abs(weights[0] - 1.0) < 5e-3
+  assert abs(weights[-1]) < 5e-3
+
+
+def test_digital_filter_initialization_removes_oscillation():
+  """Linear growth in x[0] plus a harmonic oscillator in x[1], x[2]: the
+  filter keeps the slow mode and removes the oscillation (atol 1e-3)."""
+  eq = ti.ImplicitExplicitODE.from_functions(
+      lambda x: torch.stack([torch.tensor(0.1, dtype=x.dtype), x[2], -x[1]]),
+      torch.zeros_like,
+      lambda x, eta: x)
+  dfi = ti.digital_filter_initialization(eq, ti.imex_rk_sil3, [], time_span=20.0,
+                                         cutoff_period=20.0, dt=0.01)
+  result = dfi(torch.tensor([1.0, 1.0, 1.0], dtype=torch.float64))
+  np.testing.assert_allclose(result.numpy(), [1.0, 0.0, 0.0], atol=1e-3)
+
+
+def test_time_reversed_ode_undoes_forward_steps():
+  """One SIL3 step of the reversed harmonic oscillator after one forward step
+  returns to the start to the scheme's local error."""
+  eq = ti.ImplicitExplicitODE.from_functions(lambda x: torch.stack([x[1], -x[0]]),
+                                             torch.zeros_like, lambda x, eta: x)
+  x0 = torch.tensor([1.0, 0.0], dtype=torch.float64)
+  fwd = ti.imex_rk_sil3(eq, 0.01)
+  bwd = ti.imex_rk_sil3(ti.TimeReversedImExODE(eq), 0.01)
+  x1 = bwd(fwd(x0))
+  assert (x1 - x0).abs().max().item() < 1e-8
+
+
+def test_maybe_fix_sim_time_roundoff():
+  class S:
+    sim_time = 0.30000000000000004
+  s = ti.maybe_fix_sim_time_roundoff(S(), 0.1)
+  assert abs(s.sim_time - 0.3) < 1e-15

@@ -915,7 +915,9 @@ class PrimitiveEquationsSigma:
     """Explicit tendencies (primitive_equations.py:1126-1194)."""
     validate_state_shape(state, self.coords)
     names = self._check_tracers(state)
-    if self.fused_nodal:
+    # the fused modal kernels address the Real layout's rows (m = 0, +1, -1, ...)
+    if self.fused_nodal and isinstance(self.coords.horizontal.spherical_harmonics,
+                                       spherical_harmonic.RealSphericalHarmonics):
       return self._explicit_fused(state, names)
     return self._explicit_unfused(state, names)
 

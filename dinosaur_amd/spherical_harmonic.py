@@ -454,6 +454,172 @@ def _shift_l(x, offset: int):
   return torch.nn.functional.pad(x[..., -offset:], (0, -offset))
 
 
+def _round_to_multiple(x: int, multiple: int) -> int:
+  return multiple * math.ceil(x / multiple)
+
+
+@dataclasses.dataclass(frozen=True)
+class FastSphericalHarmonics(SphericalHarmonics):
+  """The reference's "fast" modal layout (spherical_harmonic.py:411-621):
+  rows m = [0, 0', +1, -1, +2, -2, ...] with an always-zero imaginary row 0'
+  for m = 0, so modal arrays are (2M, L), optionally padded to multiples of
+  `base_shape_multiple`.
+
+  On TPUs that layout halves the Legendre operand and shards evenly.  Here
+  the transforms run in the Real layout's kernels (the per-m GEMMs never
+  materialise a repeated P, so the layout buys nothing on this GPU): the
+  modal rows are regathered around them, which costs one extra pass over the
+  modal array per transform.  `transform_precision` as for
+  RealSphericalHarmonics ('tensorfloat32', the reference default, means FP32
+  here); `spmd_mesh` must be None (one GPU per grid).
+  """
+  spmd_mesh: Any = None
+  base_shape_multiple: int | None = None
+  reverse_einsum_arg_order: bool | None = None
+  stacked_fourier_transforms: bool | None = None
+  transform_precision: str = 'tensorfloat32'
+
+  def __post_init__(self):
+    if self.spmd_mesh is not None:
+      raise ValueError('dinosaur_amd runs one grid per GPU; spmd_mesh is not supported')
+    if self.base_shape_multiple is None:
+      object.__setattr__(self, 'base_shape_multiple', 1)
+    if self.reverse_einsum_arg_order is None:
+      object.__setattr__(self, 'reverse_einsum_arg_order', False)
+    if self.stacked_fourier_transforms is None:
+      unstacked = math.ceil(self.longitude_wavenumbers / 128)
+      stacked = 2 * math.ceil(self.longitude_wavenumbers / 256)
+      object.__setattr__(self, 'stacked_fourier_transforms', stacked <= unstacked)
+    if self._precision_name not in _lib.PRECISIONS:
+      raise ValueError(f'unknown transform_precision {self.transform_precision!r}')
+
+  @property
+  def _precision_name(self):
+    return {'tensorfloat32': 'float32', 'highest': 'float32', 'default': 'float32'}.get(
+        self.transform_precision, self.transform_precision)
+
+  @functools.cached_property
+  def _real(self) -> RealSphericalHarmonics:
+    return RealSphericalHarmonics(self.longitude_wavenumbers, self.total_wavenumbers,
+                                  self.longitude_nodes, self.latitude_nodes,
+                                  self.latitude_spacing, transform_precision=self._precision_name)
+
+  @functools.cached_property
+  def nodal_limits(self):
+    return (self.longitude_nodes, self.latitude_nodes)
+
+  @functools.cached_property
+  def modal_limits(self):
+    return (2 * self.longitude_wavenumbers, self.total_wavenumbers)
+
+  @functools.cached_property
+  def nodal_shape(self):
+    b = self.base_shape_multiple
+    return tuple(_round_to_multiple(x, k) for x, k in zip(self.nodal_limits, (b, b)))
+
+  @functools.cached_property
+  def modal_shape(self):
+    b = self.base_shape_multiple
+    return tuple(_round_to_multiple(x, k) for x, k in zip(self.modal_limits, (2 * b, b)))
+
+  @functools.cached_property
+  def nodal_padding(self):
+    return tuple(x - y for x, y in zip(self.nodal_shape, self.nodal_limits))
+
+  @functools.cached_property
+  def modal_padding(self):
+    return tuple(x - y for x, y in zip(self.modal_shape, self.modal_limits))
+
+  @functools.cached_property
+  def nodal_axes(self):
+    px, py = self.nodal_padding
+    lon, sin_lat = self._real.nodal_axes
+    return np.pad(lon, [(0, px)]), np.pad(sin_lat, [(0, py)])
+
+  @functools.cached_property
+  def modal_axes(self):
+    px, py = self.modal_padding
+    m_pos = np.arange(1, self.longitude_wavenumbers)
+    m_pos_neg = np.stack([m_pos, -m_pos], axis=1).ravel()
+    return (np.pad(np.concatenate([[0, 0], m_pos_neg]), [(0, px)]),
+            np.pad(np.arange(self.total_wavenumbers), [(0, py)]))
+
+  @functools.cached_property
+  def modal_dtype(self):
+    return np.dtype(np.float32)
+
+  @functools.cached_property
+  def mask(self):
+    m, l = np.meshgrid(*self.modal_axes, indexing='ij')
+    i, j = np.meshgrid(*(np.arange(s) for s in self.modal_shape), indexing='ij')
+    i_lim, j_lim = self.modal_limits
+    return (abs(m) <= l) & (i != 1) & (i < i_lim) & (j < j_lim)
+
+  def plan(self, device) -> Plan:
+    return self._real.plan(device)
+
+  @functools.cached_property
+  def basis(self) -> _SphericalHarmonicBasis:
+    """Dense (f, p, w) in this layout (P not repeated over +-m)."""
+    b = self._real.basis
+    px, py = self.nodal_padding
+    mx, my = self.modal_padding
+    M = self.longitude_wavenumbers
+    f = np.zeros((self.longitude_nodes, 2 * M))
+    f[:, 0] = b.f[:, 0]
+    f[:, 2:] = b.f[:, 1:]
+    f = np.pad(f, [(0, px), (0, mx)])
+    if self.stacked_fourier_transforms:
+      f = np.reshape(f, (-1, 2, f.shape[-1] // 2), order='F')
+    p = b.p[0::2]  # rows 0, +1, +2, ... of the repeated Real table: one per m
+    p = np.pad(p, [(0, mx // 2), (0, py), (0, my)])
+    w = np.pad(b.w, [(0, py)])
+    return _SphericalHarmonicBasis(f=f, p=p, w=w)
+
+  @functools.lru_cache(maxsize=8)
+  def _rows(self, device):
+    """Real row r of the fast layout's row 0, 2, 3, ..., 2M-1 (row 1 is 0')."""
+    R = 2 * self.longitude_wavenumbers - 1
+    real_rows = torch.arange(R, device=device)
+    fast_of_real = torch.where(real_rows == 0, real_rows, real_rows + 1)
+    return fast_of_real
+
+  def inverse_transform(self, x):
+    """modal (..., 2M(+pad), L(+pad)) -> nodal (..., I(+pad), J(+pad))."""
+    if tuple(x.shape[-2:]) != self.modal_shape:
+      raise ValueError(f'expected modal shape (..., {self.modal_shape}), got {tuple(x.shape)}')
+    L = self.total_wavenumbers
+    xr = torch.index_select(x[..., :, :L], -2, self._rows(x.device))
+    y = self._real.inverse_transform(xr)
+    px, py = self.nodal_padding
+    return torch.nn.functional.pad(y, (0, py, 0, px)) if px or py else y
+
+  def transform(self, x):
+    """nodal (..., I(+pad), J(+pad)) -> modal (..., 2M(+pad), L(+pad))."""
+    if tuple(x.shape[-2:]) != self.nodal_shape:
+      raise ValueError(f'expected nodal shape (..., {self.nodal_shape}), got {tuple(x.shape)}')
+    I, J = self.nodal_limits
+    y = self._real.transform(x[..., :I, :J])
+    out = y.new_zeros(y.shape[:-2] + self.modal_shape)
+    out = out.index_copy(-2, self._rows(y.device),
+                         torch.nn.functional.pad(y, (0, self.modal_padding[1])))
+    return out
+
+  def longitudinal_derivative(self, x):
+    """u_x[2m] = m u[2m+1], u_x[2m+1] = -m u[2m] (fourier.py:112-126)."""
+    n = x.shape[-2]
+    if n % 2:
+      raise ValueError(f'{x.shape=} along axis=-2 is not even')
+    i = torch.arange(n, device=x.device)
+    m = (i // 2).to(x.dtype)
+    partner = torch.where(i % 2 == 0, i + 1, i - 1)
+    coef = torch.where(i % 2 == 0, m, -m)
+    return torch.index_select(x, -2, partner) * coef[:, None]
+
+
+RealSphericalHarmonicsWithZeroImag = FastSphericalHarmonics
+
+
 # ----------------------------------------------------------------- Grid
 
 SphericalHarmonicsImpl = Callable[..., SphericalHarmonics]

@@ -58,6 +58,30 @@ __device__ __forceinline__ double ldd(const T* p) {
 //   delta_bar = W^T z_bar, T_bar = dT - eta lam (W G)^T z_bar,
 //   lnps_bar = dp - eta lam w . z_bar
 // -- the tables are swapped by the host and eta / eta lam trade places.
+// Below the triangle: for the Real layout (odd R, tri != 0) the column c of a
+// member is (row r, degree l) with zonal wavenumber m = (r + 1) / 2, and
+// fields vanish for l < m.  A wave whose 16 columns all lie there has zero
+// input and zero output, so it reads nothing.
+__device__ __forceinline__ bool below_triangle(int tri, int64_t c, int64_t cols, int L) {
+  bool lane_tri = true;
+  if (c < cols) {
+    const int r = (int)(c / L), l = (int)(c - (int64_t)r * L);
+    lane_tri = l < ((r + 1) >> 1);
+  }
+  return tri && __all(lane_tri);
+}
+
+template <typename T>
+__device__ __forceinline__ void zero_columns(T* div_out, T* temp_out, T* lnps_out, int64_t base,
+                                             int64_t lcol, int64_t cols, int K, int kg, bool ok) {
+  if (!ok) return;
+  for (int k = kg; k < K; k += 4) {
+    div_out[base + (int64_t)k * cols] = (T)0;
+    temp_out[base + (int64_t)k * cols] = (T)0;
+  }
+  if (kg == 0) lnps_out[lcol] = (T)0;
+}
+
 // In place is allowed (outputs equal to inputs): a wave reads every level of
 // its 16 columns before it writes any, and no other wave touches them.
 template <typename T, int NS, bool TRANS>
@@ -66,7 +90,8 @@ __global__ __launch_bounds__(256) void solve_mfma_kernel(
     const double* __restrict__ t1a, const double* __restrict__ t1b,
     const double* __restrict__ t2, const double* __restrict__ wvec,
     const double* __restrict__ dvec, const double* __restrict__ lam, int K, int L,
-    int64_t cols, int64_t members, double eta) {
+    int64_t cols, int64_t members, double eta, int tri, int inplace) {
+  constexpr bool ACCUM = false;
   using S = Shape<NS>;
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -78,6 +103,12 @@ __global__ __launch_bounds__(256) void solve_mfma_kernel(
   const bool cok = c < cols;
   const int64_t cc = cok ? c : 0;
   const int64_t base = b * (int64_t)K * cols + cc;
+  if (below_triangle(tri, c, cols, L)) {
+    // every column of the wave has l < m: zero in, zero out
+    if (!inplace && !ACCUM) zero_columns<T>(div_out, temp_out, lnps_out, base, b * cols + c, cols,
+                                            K, kg, c < cols);
+    return;
+  }
   const double lam_l = lam[(int)(cc % L)];
   const double lp = ldd(lnps + b * cols + cc);
   // stage-1 B fragments: delta and T rows 4 s + kg (zero beyond K)
@@ -152,7 +183,8 @@ __global__ __launch_bounds__(256) void terms_mfma_kernel(
     const T* div, const T* temp, const T* lnps, T* div_out, T* temp_out, T* lnps_out,
     const double* __restrict__ tg, const double* __restrict__ th,
     const double* __restrict__ rtref, const double* __restrict__ lam, int K, int L,
-    int64_t cols, int64_t members) {
+    int64_t cols, int64_t members, int tri, int inplace) {
+  constexpr bool ACCUM = ACC;  // adding the zero result changes nothing
   using S = Shape<NS>;
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -164,6 +196,12 @@ __global__ __launch_bounds__(256) void terms_mfma_kernel(
   const bool cok = c < cols;
   const int64_t cc = cok ? c : 0;
   const int64_t base = b * (int64_t)K * cols + cc;
+  if (below_triangle(tri, c, cols, L)) {
+    // every column of the wave has l < m: zero in, zero out
+    if (!inplace && !ACCUM) zero_columns<T>(div_out, temp_out, lnps_out, base, b * cols + c, cols,
+                                            K, kg, c < cols);
+    return;
+  }
   const double lam_l = lam[(int)(cc % L)];
   const double lp = ldd(lnps + b * cols + cc);
   double xd[NS], xt[NS];
@@ -329,23 +367,28 @@ static int launch_ns(bool solve, bool adjoint, bool acc, const dino_implicit* f,
   T* y1 = (T*)t_o;
   T* y2 = (T*)l_o;
   double* const* t = f->mtab;
+  const int tri = (f->R % 2 == 1) ? 1 : 0;  // Real layout (odd row count)
+  const int inplace = (d_o == div && t_o == temp && l_o == lnps) ? 1 : 0;
   if (solve && !adjoint) {
     solve_mfma_kernel<T, NS, false><<<blocks, 256, 0, s>>>(
         x0, x1, x2, y0, y1, y2, t[0], t[1], t[2], t[3], t[4], f->lam, f->K, f->L, cols,
-        members, f->eta);
+        members, f->eta, tri, inplace);
   } else if (solve) {
     solve_mfma_kernel<T, NS, true><<<blocks, 256, 0, s>>>(
         x0, x1, x2, y0, y1, y2, t[8], t[9], t[10], t[11], t[4], f->lam, f->K, f->L, cols,
-        members, f->eta);
+        members, f->eta, tri, inplace);
   } else if (!adjoint) {
     terms_mfma_kernel<T, NS, false, false><<<blocks, 256, 0, s>>>(
-        x0, x1, x2, y0, y1, y2, t[5], t[6], t[7], f->lam, f->K, f->L, cols, members);
+        x0, x1, x2, y0, y1, y2, t[5], t[6], t[7], f->lam, f->K, f->L, cols, members, tri,
+        inplace);
   } else if (!acc) {
     terms_mfma_kernel<T, NS, true, false><<<blocks, 256, 0, s>>>(
-        x0, x1, x2, y0, y1, y2, t[12], t[13], t[14], f->lam, f->K, f->L, cols, members);
+        x0, x1, x2, y0, y1, y2, t[12], t[13], t[14], f->lam, f->K, f->L, cols, members, tri,
+        inplace);
   } else {
     terms_mfma_kernel<T, NS, true, true><<<blocks, 256, 0, s>>>(
-        x0, x1, x2, y0, y1, y2, t[12], t[13], t[14], f->lam, f->K, f->L, cols, members);
+        x0, x1, x2, y0, y1, y2, t[12], t[13], t[14], f->lam, f->K, f->L, cols, members, tri,
+        inplace);
   }
   DINO_LAUNCH_CHECK("implicit mfma kernel");
   return DINO_OK;

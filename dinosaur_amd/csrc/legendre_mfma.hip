@@ -366,7 +366,7 @@ int inv_by_width(const dino_plan* pl, const float* in, float* out, int64_t B, bo
 }  // namespace
 
 int legendre_forward_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
-                          bool split, hipStream_t s);
+                          bool split, bool triangle_only, hipStream_t s);
 int legendre_inverse_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
                           bool split, hipStream_t s);
 
@@ -391,14 +391,14 @@ int legendre_inverse_mfma(const dino_plan* pl, const void* modal, void* fourier,
 }
 
 int legendre_forward_mfma(const dino_plan* pl, const void* fourier, void* modal, int64_t B,
-                          int precision, hipStream_t s) {
+                          int precision, bool triangle_only, hipStream_t s) {
   if (B == 0) return DINO_OK;
   const float* in = static_cast<const float*>(fourier);
   float* out = static_cast<float*>(modal);
   const int fam = pl->sched[DINO_STAGE_LEGENDRE_FORWARD] & 0xff;
   if (pl->fks > 0 && (fam == DINO_SCHED_AUTO || fam == DINO_SCHED_ROWS) &&
       (precision == DINO_PREC_NATIVE || precision == DINO_PREC_BF16X3))
-    return legendre_forward_rows(pl, in, out, B, precision == DINO_PREC_BF16X3, s);
+    return legendre_forward_rows(pl, in, out, B, precision == DINO_PREC_BF16X3, triangle_only, s);
   DINO_REQUIRE((2 * B + TR - 1) / TR <= 65535, DINO_EINVAL, "batch too large");
   switch (precision) {
     case DINO_PREC_NATIVE:

@@ -81,7 +81,7 @@ template <bool BF, int KS, bool VIN, bool VOUT>
 __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
     const float* __restrict__ fourier, float* __restrict__ modal,
     const float4* __restrict__ ftab, const int64_t* __restrict__ ftile_off, int B, int L,
-    int J, int Jh, int Rrows, int mpair) {
+    int J, int Jh, int Rrows, int mpair, int skip_zero) {
   constexpr int KT = 2 * KS;  // 16-node groups
   constexpr int TF4 = tile_f4(KS);
   constexpr int PER = TF4 / 256;
@@ -197,8 +197,8 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
     }
   }
 
-  // Zero block: whole 32-degree blocks below m.
-  if (lt0 > 0) {
+  // Zero block: whole 32-degree blocks below m (left untouched on request).
+  if (lt0 > 0 && !skip_zero) {
     if constexpr (VOUT) {
       const int n4 = lt0 / 4;
       for (int idx = tid; idx < kRowsPerBlock * n4; idx += 256) {
@@ -352,7 +352,8 @@ __global__ void forward_fragments_kernel(int L, int Jh, int Jhp, int ks,
 }
 
 template <bool BF, int KS>
-int launch_rows(const dino_plan* pl, const float* in, float* out, int64_t B, hipStream_t s) {
+int launch_rows(const dino_plan* pl, const float* in, float* out, int64_t B, int skip_zero,
+                hipStream_t s) {
   const bool vin = pl->J % 8 == 0 && reinterpret_cast<uintptr_t>(in) % 16 == 0;
   const bool vout = pl->L % 4 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0;
   const dim3 grid((unsigned)((2 * B + kRowsPerBlock - 1) / kRowsPerBlock), (unsigned)pl->M);
@@ -361,7 +362,7 @@ int launch_rows(const dino_plan* pl, const float* in, float* out, int64_t B, hip
   const int mpair = (pl->sched[DINO_STAGE_LEGENDRE_FORWARD] & DINO_SCHED_MPAIRED) ? 1 : 0;
 #define ROWS_LAUNCH(VI, VO)                                                                  \
   leg_fwd_rows_kernel<BF, KS, VI, VO><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, (int)B, \
-                                                            pl->L, pl->J, pl->Jh, pl->R, mpair)
+                                                            pl->L, pl->J, pl->Jh, pl->R, mpair, skip_zero)
   if (vin && vout) ROWS_LAUNCH(true, true);
   else if (vin) ROWS_LAUNCH(true, false);
   else if (vout) ROWS_LAUNCH(false, true);
@@ -372,12 +373,13 @@ int launch_rows(const dino_plan* pl, const float* in, float* out, int64_t B, hip
 }
 
 template <bool BF>
-int rows_by_ks(const dino_plan* pl, const float* in, float* out, int64_t B, hipStream_t s) {
+int rows_by_ks(const dino_plan* pl, const float* in, float* out, int64_t B, int skip_zero,
+               hipStream_t s) {
   switch (pl->fks) {
-    case 1: return launch_rows<BF, 1>(pl, in, out, B, s);
-    case 2: return launch_rows<BF, 2>(pl, in, out, B, s);
-    case 4: return launch_rows<BF, 4>(pl, in, out, B, s);
-    case 8: return launch_rows<BF, 8>(pl, in, out, B, s);
+    case 1: return launch_rows<BF, 1>(pl, in, out, B, skip_zero, s);
+    case 2: return launch_rows<BF, 2>(pl, in, out, B, skip_zero, s);
+    case 4: return launch_rows<BF, 4>(pl, in, out, B, skip_zero, s);
+    case 8: return launch_rows<BF, 8>(pl, in, out, B, skip_zero, s);
     default:
       set_error("forward fragment tables hold %d steps", pl->fks);
       return DINO_EINVAL;
@@ -415,10 +417,11 @@ void forward_fragment_fill(const dino_plan* pl, hipStream_t s) {
 bool forward_rows_available(const dino_plan* pl) { return pl->fks > 0; }
 
 int legendre_forward_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
-                          bool split, hipStream_t s) {
+                          bool split, bool triangle_only, hipStream_t s) {
   if (B == 0) return DINO_OK;
   DINO_REQUIRE(2 * B / kRowsPerBlock < (int64_t)1 << 30, DINO_EINVAL, "batch too large");
-  return split ? rows_by_ks<true>(pl, in, out, B, s) : rows_by_ks<false>(pl, in, out, B, s);
+  const int z = triangle_only ? 1 : 0;
+  return split ? rows_by_ks<true>(pl, in, out, B, z, s) : rows_by_ks<false>(pl, in, out, B, z, s);
 }
 
 }  // namespace dino

@@ -40,7 +40,7 @@ namespace dino {
 namespace {
 
 struct MOff {
-  int il, ev, mask, up, dn, up2, dn2, jr, pr, total;
+  int il, ev, mask, up, dn, up2, dn2, jr, pr, zero, total;
 };
 
 __host__ __device__ inline MOff moff(int R, int L) {
@@ -54,7 +54,8 @@ __host__ __device__ inline MOff moff(int R, int L) {
   o.dn2 = o.up2 + R * L;
   o.jr = o.dn2 + R * L;
   o.pr = o.jr + R;
-  o.total = o.pr + R;
+  o.zero = o.pr + R;  // R*L zeros: the field that threads below the triangle read
+  o.total = o.zero + R * L;
   return o;
 }
 
@@ -133,7 +134,10 @@ struct Stencil {
     row = r * L;
     lm = l > 0 ? l - 1 : 0;
     lp = l + 1 < L ? l + 1 : L - 1;
-    hm = l >= 1;
+    // the neighbour below counts only inside the triangle (l - 1 >= m): the
+    // degrees under it are zero, and a buffer written by a triangle-only
+    // transform leaves some of them untouched
+    hm = l - 1 >= ((r + 1) >> 1);
     hp = l + 1 < L;
     jr = c[o.jr + r];
     prow = (int)c[o.pr + r] * L;
@@ -182,12 +186,22 @@ __global__ __launch_bounds__(256) void prep_kernel(const T* __restrict__ st, T* 
   const int64_t e = blockIdx.y;
   if (pos >= p.RL) return;
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);
+  // only the inverse Legendre transform reads this output, from degree
+  // 32 * floor(m / 32) on: the degrees below are not written
+  if (l < ((r + 1) >> 1) / 32 * 32) return;
   const MOff o = moff(p.R, p.L);
   Stencil<T> S(cst, o, p.R, p.L, r, l);
   const int K = p.K;
   const T inva = (T)(1.0 / p.a);
   const T il = cst[o.il + l];
-  auto F = [&](const T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
+  // Below the triangle (l < m) every field of the Real layout is zero, and so
+  // is every output: those threads read a zero field in the constant block
+  // instead of HBM and write zeros.
+  const bool tri = l < ((r + 1) >> 1);
+  const T* zb = cst + o.zero;
+  auto F = [&](const T* base, int64_t off, int lev, int k) {
+    return tri ? zb : base + (off + e * lev + k) * p.RL;
+  };
   auto W = [&](T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
   for (int k = 0; k < K; ++k) {
     const T* z = F(st, sg.z, K, k);
@@ -231,7 +245,14 @@ __global__ __launch_bounds__(256) void prep_adjoint_kernel(const T* __restrict__
   const int K = p.K;
   const T inva = (T)(1.0 / p.a);
   const T il = cst[o.il + l];
-  auto F = [&](const T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
+  // Below the triangle (l < m) every field of the Real layout is zero, and so
+  // is every output: those threads read a zero field in the constant block
+  // instead of HBM and write zeros.
+  const bool tri = l < ((r + 1) >> 1);
+  const T* zb = cst + o.zero;
+  auto F = [&](const T* base, int64_t off, int lev, int k) {
+    return tri ? zb : base + (off + e * lev + k) * p.RL;
+  };
   auto W = [&](T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
   for (int k = 0; k < K; ++k) {
     const T* ub = F(gout, pg.u, K, k);
@@ -272,7 +293,14 @@ __global__ __launch_bounds__(256) void epilogue_kernel(const T* __restrict__ in,
   const T inva = (T)(1.0 / p.a);
   const T ev = cst[o.ev + l], mask = cst[o.mask + l];
   const T orot = oro ? -(T)p.g * ev * oro[pos] : (T)0;
-  auto F = [&](const T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
+  // Below the triangle (l < m) every field of the Real layout is zero, and so
+  // is every output: those threads read a zero field in the constant block
+  // instead of HBM and write zeros.
+  const bool tri = l < ((r + 1) >> 1);
+  const T* zb = cst + o.zero;
+  auto F = [&](const T* base, int64_t off, int lev, int k) {
+    return tri ? zb : base + (off + e * lev + k) * p.RL;
+  };
   auto W = [&](T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
   const bool moist = p.hum >= 0;
   for (int k = 0; k < K; ++k) {
@@ -310,12 +338,22 @@ __global__ __launch_bounds__(256) void epilogue_adjoint_kernel(const T* __restri
   const int64_t e = blockIdx.y;
   if (pos >= p.RL) return;
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);
+  // only the inverse Legendre transform reads this output, from degree
+  // 32 * floor(m / 32) on: the degrees below are not written
+  if (l < ((r + 1) >> 1) / 32 * 32) return;
   const MOff o = moff(p.R, p.L);
   Stencil<T> S(cst, o, p.R, p.L, r, l);
   const int K = p.K;
   const T inva = (T)(1.0 / p.a);
   const T ev = cst[o.ev + l];
-  auto F = [&](const T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
+  // Below the triangle (l < m) every field of the Real layout is zero, and so
+  // is every output: those threads read a zero field in the constant block
+  // instead of HBM and write zeros.
+  const bool tri = l < ((r + 1) >> 1);
+  const T* zb = cst + o.zero;
+  auto F = [&](const T* base, int64_t off, int lev, int k) {
+    return tri ? zb : base + (off + e * lev + k) * p.RL;
+  };
   auto W = [&](T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
   // cotangents of the clipped outputs: y' = mask * ybar (mask is diagonal),
   // read through a masked view so stencil neighbours are masked too

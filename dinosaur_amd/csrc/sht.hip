@@ -9,7 +9,7 @@ int dft_inverse_impl(const dino_plan*, const void*, void*, int64_t, int, int, hi
 int legendre_inverse_fma(const dino_plan*, const void*, void*, int64_t, int, hipStream_t);
 int legendre_forward_fma(const dino_plan*, const void*, void*, int64_t, int, hipStream_t);
 int legendre_inverse_mfma(const dino_plan*, const void*, void*, int64_t, int, hipStream_t);
-int legendre_forward_mfma(const dino_plan*, const void*, void*, int64_t, int, hipStream_t);
+int legendre_forward_mfma(const dino_plan*, const void*, void*, int64_t, int, bool, hipStream_t);
 }  // namespace dino
 
 using namespace dino;
@@ -37,10 +37,19 @@ int leg_inverse(const dino_plan* p, const void* in, void* out, int64_t B, int dt
   return legendre_inverse_mfma(p, in, out, B, prec, s);
 }
 
+// triangle_only: rows may skip the all-zero degrees below 32 * floor(m / 32)
+// (the FP64 and tiled kernels write them anyway, which is also allowed).
 int leg_forward(const dino_plan* p, const void* in, void* out, int64_t B, int dtype, int prec,
-                hipStream_t s) {
+                hipStream_t s, bool triangle_only = false) {
   if (dtype == DINO_F64) return legendre_forward_fma(p, in, out, B, dtype, s);
-  return legendre_forward_mfma(p, in, out, B, prec, s);
+  return legendre_forward_mfma(p, in, out, B, prec, triangle_only, s);
+}
+
+// Splits the DINO_SHT_TRIANGLE_ONLY flag off a precision argument.
+bool take_triangle_flag(int* precision) {
+  const bool t = (*precision & DINO_SHT_TRIANGLE_ONLY) != 0;
+  *precision &= ~DINO_SHT_TRIANGLE_ONLY;
+  return t;
 }
 
 }  // namespace
@@ -62,21 +71,23 @@ extern "C" int dino_sht_inverse(const dino_plan* p, const void* modal, void* nod
 
 extern "C" int dino_sht_forward(const dino_plan* p, const void* nodal, void* modal, void* work,
                                 int64_t batch, int dtype, int precision, void* stream) {
+  const bool tri = take_triangle_flag(&precision);
   CHECK_OR_RETURN(check_common(p, nodal, modal, batch, dtype, precision));
   DINO_REQUIRE(batch == 0 || work, DINO_EINVAL, "NULL workspace");
   hipStream_t s = as_stream(stream);
   CHECK_OR_RETURN(dft_forward_impl(p, nodal, work, batch, dtype, 1, s));
-  return leg_forward(p, work, modal, batch, dtype, precision, s);
+  return leg_forward(p, work, modal, batch, dtype, precision, s, tri);
 }
 
 extern "C" int dino_sht_inverse_adjoint(const dino_plan* p, const void* nodal_cot,
                                         void* modal_cot, void* work, int64_t batch, int dtype,
                                         int precision, void* stream) {
+  const bool tri = take_triangle_flag(&precision);
   CHECK_OR_RETURN(check_common(p, nodal_cot, modal_cot, batch, dtype, precision));
   DINO_REQUIRE(batch == 0 || work, DINO_EINVAL, "NULL workspace");
   hipStream_t s = as_stream(stream);
   CHECK_OR_RETURN(dft_forward_impl(p, nodal_cot, work, batch, dtype, 0, s));
-  return leg_forward(p, work, modal_cot, batch, dtype, precision, s);
+  return leg_forward(p, work, modal_cot, batch, dtype, precision, s, tri);
 }
 
 extern "C" int dino_sht_forward_adjoint(const dino_plan* p, const void* modal_cot,
@@ -109,6 +120,7 @@ extern "C" int dino_legendre_inverse(const dino_plan* p, const void* modal, void
 
 extern "C" int dino_legendre_forward(const dino_plan* p, const void* fourier, void* modal,
                                      int64_t batch, int dtype, int precision, void* stream) {
+  const bool tri = take_triangle_flag(&precision);
   CHECK_OR_RETURN(check_common(p, fourier, modal, batch, dtype, precision));
-  return leg_forward(p, fourier, modal, batch, dtype, precision, as_stream(stream));
+  return leg_forward(p, fourier, modal, batch, dtype, precision, as_stream(stream), tri);
 }

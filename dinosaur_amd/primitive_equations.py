@@ -932,9 +932,15 @@ class PrimitiveEquationsSigma:
     members = int(np.prod(lead)) if lead else 1
     mh = self._modal_handle(buf.device, names)
     prep = _ModalPrep.apply(buf, mh, members)
-    nodal = sh.inverse_transform(prep)
+    # the modal buffers between the transforms and the prep / epilogue kernels
+    # are read only on the triangle l >= m: written triangle-only
+    own = (type(sh).inverse_transform is spherical_harmonic.RealSphericalHarmonics.inverse_transform
+           and type(sh).transform is spherical_harmonic.RealSphericalHarmonics.transform)
+    kw_inv = {'_triangle_only_adjoint': True} if own else {}
+    kw_fwd = {'_triangle_only': True} if own else {}
+    nodal = sh.inverse_transform(prep, **kw_inv)
     nout = _NodalFused.apply(nodal, self._nodal_handle(buf.device, names), members)
-    mod = sh.transform(nout)
+    mod = sh.transform(nout, **kw_fwd)
     tend = _ModalEpilogue.apply(mod, mh, members, self._orography_modal(buf).contiguous())
     return State.from_flat(tend, lead, K, names,
                            sim_time=None if state.sim_time is None else 1.0)

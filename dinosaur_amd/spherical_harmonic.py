@@ -204,9 +204,12 @@ def sht_call(fn_name: str, plan: Plan, x: torch.Tensor, out_tail: tuple, precisi
   if batch == 0:
     return out
   work = _workspace(plan, batch, x)
+  flags = precision & _lib.SHT_TRIANGLE_ONLY
+  precision &= ~_lib.SHT_TRIANGLE_ONLY
   if x.dtype == torch.float64:
     precision = _lib.PREC_NATIVE  # FP64 data always runs FP64 arithmetic
   plan.apply_tuned(batch, _dtype_code(x), precision)
+  precision |= flags
   fn = getattr(_lib.lib(), fn_name)
   with torch.cuda.device(x.device):
     rc = fn(plan.handle, _ptr(x), _ptr(out), _ptr(work), batch, _dtype_code(x), precision,
@@ -216,33 +219,39 @@ def sht_call(fn_name: str, plan: Plan, x: torch.Tensor, out_tail: tuple, precisi
 
 
 class _ToNodal(torch.autograd.Function):
-  """modal (B, 2M-1, L) -> nodal (B, I, J); backward = P^T F^T."""
+  """modal (B, 2M-1, L) -> nodal (B, I, J); backward = P^T F^T.  partial: the
+  modal cotangent is consumed only by kernels that read the triangle l >= m,
+  so it is written triangle-only (DINO_SHT_TRIANGLE_ONLY)."""
 
   @staticmethod
-  def forward(ctx, x, plan, precision):
-    ctx.plan, ctx.precision = plan, precision
+  def forward(ctx, x, plan, precision, partial):
+    ctx.plan, ctx.precision, ctx.partial = plan, precision, partial
     return sht_call('dino_sht_inverse', plan, x, (plan.I, plan.J), precision)
 
   @staticmethod
   def backward(ctx, g):
     plan = ctx.plan
-    return (sht_call('dino_sht_inverse_adjoint', plan, g, (plan.R, plan.L), ctx.precision),
-            None, None)
+    prec = ctx.precision | (_lib.SHT_TRIANGLE_ONLY if ctx.partial else 0)
+    return (sht_call('dino_sht_inverse_adjoint', plan, g, (plan.R, plan.L), prec),
+            None, None, None)
 
 
 class _ToModal(torch.autograd.Function):
-  """nodal (B, I, J) -> modal (B, 2M-1, L); backward = w F P."""
+  """nodal (B, I, J) -> modal (B, 2M-1, L); backward = w F P.  partial: the
+  modal output is consumed only by kernels that read the triangle l >= m, so
+  it is written triangle-only."""
 
   @staticmethod
-  def forward(ctx, z, plan, precision):
+  def forward(ctx, z, plan, precision, partial):
     ctx.plan, ctx.precision = plan, precision
-    return sht_call('dino_sht_forward', plan, z, (plan.R, plan.L), precision)
+    prec = precision | (_lib.SHT_TRIANGLE_ONLY if partial else 0)
+    return sht_call('dino_sht_forward', plan, z, (plan.R, plan.L), prec)
 
   @staticmethod
   def backward(ctx, g):
     plan = ctx.plan
     return (sht_call('dino_sht_forward_adjoint', plan, g, (plan.I, plan.J), ctx.precision),
-            None, None)
+            None, None, None)
 
 
 # ----------------------------------------------------------------- array interop
@@ -397,24 +406,26 @@ class RealSphericalHarmonics(SphericalHarmonics):
     x, wp = plan.nodes()
     return _SphericalHarmonicBasis(f=f, p=p, w=(2 * np.pi / I) * wp)
 
-  def inverse_transform(self, x):
+  def inverse_transform(self, x, _triangle_only_adjoint=False):
     """modal (..., 2M-1, L) -> nodal (..., I, J)."""
     t = x
     lead = t.shape[:-2]
     if t.shape[-2:] != self.modal_shape:
       raise ValueError(f'expected modal shape (..., {self.modal_shape}), got {tuple(t.shape)}')
     plan = self.plan(t.device)
-    y = _ToNodal.apply(t.reshape((-1,) + self.modal_shape), plan, self._precision)
+    y = _ToNodal.apply(t.reshape((-1,) + self.modal_shape), plan, self._precision,
+                       _triangle_only_adjoint)
     return y.reshape(lead + self.nodal_shape)
 
-  def transform(self, x):
+  def transform(self, x, _triangle_only=False):
     """nodal (..., I, J) -> modal (..., 2M-1, L)."""
     t = x
     lead = t.shape[:-2]
     if t.shape[-2:] != self.nodal_shape:
       raise ValueError(f'expected nodal shape (..., {self.nodal_shape}), got {tuple(t.shape)}')
     plan = self.plan(t.device)
-    y = _ToModal.apply(t.reshape((-1,) + self.nodal_shape), plan, self._precision)
+    y = _ToModal.apply(t.reshape((-1,) + self.nodal_shape), plan, self._precision,
+                       _triangle_only)
     return y.reshape(lead + self.modal_shape)
 
   def longitudinal_derivative(self, x):

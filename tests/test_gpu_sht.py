@@ -196,3 +196,31 @@ def test_tl255_round_trip_config1_all_modes(gpu_device):
     rms = max(np.sqrt(((z[i] - ref[i]) ** 2).mean() / (ref[i] ** 2).mean()) for i in range(222))
     print(f'TL255 round trip {prec}: worst max-rel {mx:.2e}, worst rms-rel {rms:.2e}', flush=True)
     assert mx < tol and rms < tol, (prec, mx, rms)
+
+
+@pytest.mark.parametrize('fn', ['dino_sht_forward', 'dino_sht_inverse_adjoint'])
+def test_triangle_only_flag_leaves_zero_block_untouched(fn, gpu_device):
+  """DINO_SHT_TRIANGLE_ONLY: each modal row of wavenumber m is written from
+  degree 32 * floor(m / 32) on, bitwise equal to the full call there, and the
+  degrees below keep what the buffer held (a NaN sentinel here).  TL127, FP32,
+  6 fields."""
+  import ctypes
+  from dinosaur_amd import _lib
+  print(f'[test] triangle-only flag {fn}', flush=True)
+  g = sh.Grid.TL127()
+  plan = g.spherical_harmonics.plan(gpu_device)
+  rs = np.random.RandomState(9)
+  z = torch.tensor(rs.normal(size=(6,) + g.nodal_shape), dtype=torch.float32, device=gpu_device)
+  full = sh.sht_call(fn, plan, z, (plan.R, plan.L), _lib.PREC_NATIVE)
+  out = torch.full((6, plan.R, plan.L), float('nan'), device=gpu_device)
+  work = torch.empty((6, plan.R, plan.J), device=gpu_device)
+  p = lambda t: ctypes.c_void_p(t.data_ptr())
+  stream = ctypes.c_void_p(torch.cuda.current_stream(gpu_device).cuda_stream)
+  rc = getattr(_lib.lib(), fn)(plan.handle, p(z), p(out), p(work), 6, _lib.DINO_F32,
+                               _lib.PREC_NATIVE | _lib.SHT_TRIANGLE_ONLY, stream)
+  _lib.check(rc, fn)
+  torch.cuda.synchronize()
+  for r in range(plan.R):
+    lt0 = 32 * (((r + 1) // 2) // 32)
+    assert torch.isnan(out[:, r, :lt0]).all(), r
+    assert torch.equal(out[:, r, lt0:], full[:, r, lt0:]), r

@@ -53,6 +53,8 @@ struct StageArgs {
   int n[2];
   T dt;
   int64_t ib, ie, go;
+  int64_t trl;  // > 0: Real-layout modal fields of trl = R * L elements ...
+  int tl;       // ... of tl degrees per row; elements with l < m are zero
 };
 
 template <typename T>
@@ -64,7 +66,19 @@ struct TargetArgs {
   int n;
   T dt;
   int64_t ib, ie, go;
+  int64_t trl;
+  int tl;
 };
+
+// Elements i .. i + w - 1 all lie below the triangle (l < m, m = (r + 1) / 2
+// for row r of the Real layout) of their field.  The triangle is a prefix of
+// each row and m < L, so it suffices that the first element's l + w - 1 < m.
+__device__ __forceinline__ bool below_triangle(int64_t i, int w, int64_t trl, int tl) {
+  if (trl <= 0) return false;
+  const int64_t j = i % trl;
+  const int r = (int)(j / tl), l = (int)(j - (int64_t)r * tl);
+  return l + w - 1 < ((r + 1) >> 1);
+}
 
 template <typename T>
 struct AdjointArgs {
@@ -89,6 +103,7 @@ __device__ __forceinline__ U group_sum(const StageArgs<T>& a, int grp, int64_t i
 
 template <typename T>
 __device__ __forceinline__ T combine_scalar(const StageArgs<T>& a, int64_t i) {
+  if (below_triangle(i, 1, a.trl, a.tl)) return (T)0;
   T y = a.y0[i];
   if (a.n[0]) y = y + a.dt * group_sum<T, T>(a, 0, i);
   if (a.n[1] && i >= a.ib && i < a.ie) y = y + a.dt * group_sum<T, T>(a, 1, i - a.go);
@@ -109,6 +124,10 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a, T* __restric
   if (k < nv) {
     if constexpr (VEC) {
       const int64_t i0 = k * W;
+      if (below_triangle(i0, W, a.trl, a.tl)) {  // every operand is zero there
+        reinterpret_cast<V*>(out)[k] = V{};
+        return;
+      }
       V y = reinterpret_cast<const V*>(a.y0)[k];
       if (a.n[0]) y = y + a.dt * group_sum<T, V>(a, 0, k);
       if (a.n[1]) {
@@ -158,13 +177,15 @@ __global__ __launch_bounds__(256) void stage_adjoint_kernel(AdjointArgs<T> a, in
 
 template <typename T>
 __device__ __forceinline__ void target_scalar(const TargetArgs<T>& a, int64_t i) {
-  const T g = a.g[i];
+  const bool tri = below_triangle(i, 1, a.trl, a.tl);
+  const T g = tri ? (T)0 : a.g[i];
   const T t = a.dt * g;
 #pragma unroll
   for (int j = 0; j < kMaxTargets; ++j) {
     if (j >= a.n) break;
     const int kind = a.kind[j];
     const T v = (kind & DINO_TARGET_IDENTITY) ? g : a.c[j] * t;
+    if (tri && (kind & DINO_TARGET_ACCUMULATE)) continue;  // adds zero
     if (kind & DINO_TARGET_RANGED) {
       if (i < a.ib || i >= a.ie) continue;
       T* o = a.out[j] + (i - a.go);
@@ -185,7 +206,8 @@ __global__ __launch_bounds__(256) void targets_kernel(TargetArgs<T> a, int64_t n
   if (k < nv) {
     if constexpr (VEC) {
       const int64_t i0 = k * W;
-      const V g = reinterpret_cast<const V*>(a.g)[k];
+      const bool tri = below_triangle(i0, W, a.trl, a.tl);
+      const V g = tri ? V{} : reinterpret_cast<const V*>(a.g)[k];
       const V t = a.dt * g;
       const bool inside = i0 >= a.ib && i0 + W <= a.ie;
       const bool touch = i0 + W > a.ib && i0 < a.ie;
@@ -195,6 +217,7 @@ __global__ __launch_bounds__(256) void targets_kernel(TargetArgs<T> a, int64_t n
         const int kind = a.kind[j];
         const bool acc = (kind & DINO_TARGET_ACCUMULATE) != 0;
         const V v = (kind & DINO_TARGET_IDENTITY) ? g : a.c[j] * t;
+        if (tri && acc) continue;  // adds zero
         if (!(kind & DINO_TARGET_RANGED)) {
           V* o = reinterpret_cast<V*>(a.out[j]) + k;
           *o = acc ? *o + v : v;
@@ -225,8 +248,11 @@ inline bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 1
 template <typename T>
 int launch_stage(const void* y0, int n_ex, const void* const* f, const double* a, int n_im,
                  const void* const* g, const double* b, double dt, void* out, int64_t n,
-                 int64_t ib, int64_t ie, int64_t go, hipStream_t s) {
+                 int64_t ib, int64_t ie, int64_t go, hipStream_t s, int64_t trl = 0,
+                 int tl = 0) {
   StageArgs<T> args{};
+  args.trl = trl;
+  args.tl = tl;
   args.y0 = static_cast<const T*>(y0);
   args.n[0] = n_ex;
   args.n[1] = n_im;
@@ -290,8 +316,10 @@ int launch_adjoint(const void* gout, int n_ex, const double* a, int n_im, const 
 template <typename T>
 int launch_targets(const void* gbar, int nt, void* const* outp, const double* coef,
                    const int* kind, double dt, int64_t n, int64_t ib, int64_t ie, int64_t go,
-                   hipStream_t s) {
+                   hipStream_t s, int64_t trl, int tl) {
   TargetArgs<T> args{};
+  args.trl = trl;
+  args.tl = tl;
   args.g = static_cast<const T*>(gbar);
   args.n = nt;
   args.dt = (T)dt;
@@ -361,8 +389,13 @@ extern "C" int dino_imex_stage_ranged(const void* y0, int n_ex, const void* cons
                                       const double* a, int n_im, const void* const* g,
                                       const double* b, double dt, void* out, int64_t n,
                                       int64_t im_begin, int64_t im_end, int64_t g_origin,
-                                      int dtype, void* stream) {
+                                      int64_t field_size, int degrees, int dtype,
+                                      void* stream) {
   const char* name = "dino_imex_stage_ranged";
+  DINO_REQUIRE(degrees == 0 || (degrees > 0 && field_size > 0 && field_size % degrees == 0 &&
+                                (field_size / degrees) % 2 == 1 && n % field_size == 0),
+               DINO_EINVAL, "%s: invalid Real-layout field shape (%lld, %d)", name,
+               (long long)field_size, degrees);
   int rc = check_args(name, n_ex, n_im, a, b, n, dtype);
   if (rc != DINO_OK) return rc;
   rc = check_range(name, n, im_begin, im_end, g_origin);
@@ -374,17 +407,22 @@ extern "C" int dino_imex_stage_ranged(const void* y0, int n_ex, const void* cons
   if (n == 0) return DINO_OK;
   if (dtype == DINO_F64)
     return launch_stage<double>(y0, n_ex, f, a, n_im, g, b, dt, out, n, im_begin, im_end,
-                                g_origin, as_stream(stream));
+                                g_origin, as_stream(stream), degrees ? field_size : 0, degrees);
   return launch_stage<float>(y0, n_ex, f, a, n_im, g, b, dt, out, n, im_begin, im_end,
-                             g_origin, as_stream(stream));
+                             g_origin, as_stream(stream), degrees ? field_size : 0, degrees);
 }
 
 extern "C" int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
                                                void* const* out, const double* coef,
                                                const int* kind, double dt, int64_t n,
                                                int64_t im_begin, int64_t im_end,
-                                               int64_t g_origin, int dtype, void* stream) {
+                                               int64_t g_origin, int64_t field_size,
+                                               int degrees, int dtype, void* stream) {
   const char* name = "dino_imex_stage_adjoint_targets";
+  DINO_REQUIRE(degrees == 0 || (degrees > 0 && field_size > 0 && field_size % degrees == 0 &&
+                                (field_size / degrees) % 2 == 1 && n % field_size == 0),
+               DINO_EINVAL, "%s: invalid Real-layout field shape (%lld, %d)", name,
+               (long long)field_size, degrees);
   DINO_REQUIRE(n_targets >= 0 && n_targets <= kMaxTargets, DINO_EINVAL,
                "%s: at most %d targets (got %d)", name, kMaxTargets, n_targets);
   DINO_REQUIRE(gbar && (n_targets == 0 || (out && coef && kind)), DINO_EINVAL,
@@ -403,9 +441,10 @@ extern "C" int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
   if (n == 0 || n_targets == 0) return DINO_OK;
   if (dtype == DINO_F64)
     return launch_targets<double>(gbar, n_targets, out, coef, kind, dt, n, im_begin, im_end,
-                                  g_origin, as_stream(stream));
+                                  g_origin, as_stream(stream), degrees ? field_size : 0,
+                                  degrees);
   return launch_targets<float>(gbar, n_targets, out, coef, kind, dt, n, im_begin, im_end,
-                               g_origin, as_stream(stream));
+                               g_origin, as_stream(stream), degrees ? field_size : 0, degrees);
 }
 
 extern "C" int dino_imex_stage_adjoint(const void* out_cot, int n_ex, const double* a,

@@ -595,6 +595,11 @@ class _FlatImexOps:
     self.go = self.ib - self.ib % 4
     self.n = (3 * E * K + E + E * K * len(names)) * self.RL
     self.op = eq._implicit
+    # Real layout: the stage kernels skip the all-zero part l < m of each field
+    real = isinstance(eq.coords.horizontal.spherical_harmonics,
+                      spherical_harmonic.RealSphericalHarmonics)
+    self.field_size = self.RL if real else 0
+    self.degrees = L if real else 0
 
   def state(self, buf):
     return State.from_flat(buf, *self.layout)

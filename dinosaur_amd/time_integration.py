@@ -222,7 +222,7 @@ def _fused_stage(ops, y0, dt, ex, im):
   rc = _lib.lib().dino_imex_stage_ranged(
       ctypes.c_void_p(y0.data_ptr()), len(fs), _ptrs(fs), _dbls([a for a, _ in ex]), len(gs),
       _ptrs(gs), _dbls([b for b, _ in im]), float(dt), ctypes.c_void_p(out.data_ptr()),
-      y0.numel(), ops.ib, ops.ie, ops.go, _code(y0), stream)
+      y0.numel(), ops.ib, ops.ie, ops.go, ops.field_size, ops.degrees, _code(y0), stream)
   _lib.check(rc, 'dino_imex_stage_ranged')
   return out
 
@@ -237,7 +237,7 @@ def _fused_targets(ops, gbar, dt, targets):
   rc = _lib.lib().dino_imex_stage_adjoint_targets(
       ctypes.c_void_p(gbar.data_ptr()), len(targets), _ptrs([b for b, _, _ in targets]),
       _dbls([c for _, c, _ in targets]), kinds, float(dt), gbar.numel(), ops.ib, ops.ie,
-      ops.go, _code(gbar), stream)
+      ops.go, ops.field_size, ops.degrees, _code(gbar), stream)
   _lib.check(rc, 'dino_imex_stage_adjoint_targets')
 
 

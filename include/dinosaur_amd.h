@@ -257,12 +257,18 @@ int dino_imex_stage_adjoint(const void* out_cot, int n_ex, const double* a,
  * adds dt * b * 0 there: the same value).  For the primitive equations the
  * range is the divergence, temperature and ln ps part of the flat state:
  * vorticity and tracers have no implicit tendency
- * (dinosaur/primitive_equations.py:1230-1249).  out may alias y0. */
+ * (dinosaur/primitive_equations.py:1230-1249).  out may alias y0.
+ * degrees > 0 declares the buffers whole Real-layout modal fields of
+ * field_size = (2M - 1) * degrees elements: entries with l < m (the part of
+ * the dense array outside the triangular truncation, zero in every operand
+ * of a masked state) are written as zero without reading any operand.
+ * degrees = 0: plain elementwise. */
 int dino_imex_stage_ranged(const void* y0, int n_ex, const void* const* f,
                            const double* a, int n_im, const void* const* g,
                            const double* b, double dt, void* out, int64_t n,
                            int64_t im_begin, int64_t im_end, int64_t g_origin,
-                           int dtype, void* stream);
+                           int64_t field_size, int degrees, int dtype,
+                           void* stream);
 
 /* Cotangent targets of one stage combination, written in one pass over the
  * output cotangent gbar: for each target t,
@@ -275,7 +281,9 @@ int dino_imex_stage_ranged(const void* y0, int n_ex, const void* const* f,
  * the reverse sweep of imex_runge_kutta (time_integration.py:351-370) with
  * the accumulation of each term's cotangent over the stages that use it
  * fused into the pass.  At most 9 targets; out buffers must not overlap
- * gbar or each other. */
+ * gbar or each other.  field_size / degrees as for dino_imex_stage_ranged:
+ * below the triangle gbar is not read, written targets get zero and
+ * accumulating ones are left as they are. */
 enum {
   DINO_TARGET_ACCUMULATE = 1,
   DINO_TARGET_IDENTITY = 2,
@@ -285,7 +293,8 @@ int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
                                     void* const* out, const double* coef,
                                     const int* kind, double dt, int64_t n,
                                     int64_t im_begin, int64_t im_end,
-                                    int64_t g_origin, int dtype, void* stream);
+                                    int64_t g_origin, int64_t field_size,
+                                    int degrees, int dtype, void* stream);
 
 /* Fused nodal nonlinear terms of PrimitiveEquationsSigma.explicit_terms.
  * Replaces compute_diagnostic_state_sigma and the nodal parts of

@@ -428,8 +428,10 @@ def test_fused_imex_step_matches_stagewise_autograd(dtype, gpu_device):
     yf, ys = step_f(x0), step_s(x0)
   assert torch.equal(yf.flat_buffer(), ys.flat_buffer())
   assert yf.sim_time == ys.sim_time
-  w = torch.as_tensor(rs.normal(size=tuple(x0.flat_buffer().shape)), device=gpu_device,
-                      dtype=dtype)
+  # a physical cotangent: entries outside the triangular truncation (l < |m|)
+  # are structurally zero in the fused kernels and carry no weight
+  w = torch.as_tensor(rs.normal(size=tuple(x0.flat_buffer().shape)) * grid.mask,
+                      device=gpu_device, dtype=dtype)
 
   def grad_of(step):
     leaf = x0.flat_buffer().clone().requires_grad_(True)
@@ -471,3 +473,27 @@ def test_digital_filter_initialization_damps_gravity_waves(gpu_device):
   before, after = div_tendency(x0), div_tendency(x1)
   print(f'  divergence tendency rms {before:.3e} -> {after:.3e}', flush=True)
   assert after < before / 10
+
+
+def test_entries_below_triangle_are_structurally_zero(gpu_device):
+  """Values planted at l < |m| (outside the triangular truncation) never reach
+  the physical entries -- in the reference either: the transforms, derivative
+  recurrences and per-(m, l) implicit solve do not couple them in.  The fused
+  kernels do not carry them (what the step leaves there is unspecified);
+  physical entries equal the step of the masked state bitwise.  Moist T21 L5,
+  FP64."""
+  print('[test] below-triangle entries', flush=True)
+  grid = sh.Grid.T21()
+  K = 5
+  coords = coordinate_systems.CoordinateSystem(grid, sigma_coordinates.SigmaCoordinates.equidistant(K))
+  eq = pe.PrimitiveEquationsSigma(np.linspace(220., 290., K), np.zeros(grid.modal_shape), coords,
+                                  units.SimUnits.from_si(), humidity_key='q')
+  st = pe.to_flat(_random_state(coords, ['q'], gpu_device, 51))
+  mask = torch.as_tensor(grid.mask, device=gpu_device)
+  noisy = st.flat_buffer().clone()
+  noisy += torch.randn_like(noisy) * (~mask)
+  step = ti.imex_rk_sil3(eq, 0.01)
+  with torch.no_grad():
+    a = step(st).flat_buffer()
+    b = step(pe.State.from_flat(noisy, *st._flat[1:])).flat_buffer()
+  assert torch.equal(a * mask, b * mask)

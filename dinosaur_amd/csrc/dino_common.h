@@ -79,6 +79,15 @@ struct dino_plan {
   int sched[4];
 };
 
+// Orders this wave's LDS writes before its later LDS reads when the staging
+// region is private to the wave: no workgroup barrier, so the waves of a
+// block need not wait for each other.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Zonal wavenumber handled by grid row y of a Legendre launch over M rows:
 // natural order (longest triangular slab first) or paired (0, M-1, 1, M-2,
 // ...), which alternates long and short slabs.

@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
           if (jj < Jh && pos < ndn) os[(4 * g + i) * OPITCH + pos] = e - o;
         }
       }
-    __syncthreads();
+    wave_lds_sync();  // the output staging rows are private to the wave
     const int col0 = half == 0 ? J - Jh : 0;
     if constexpr (VOUT) {
       const int n4 = len / 4;  // J % 8 == 0: both halves hold Jh floats
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
         }
       }
     }
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 

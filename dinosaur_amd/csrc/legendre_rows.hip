@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
         if (rho < 16)
           *reinterpret_cast<floatx4*>(stg + (wave * 16 + rho) * PITCH + 4 * c) = u[it] + sp * d[it];
       }
-      __syncthreads();
+      wave_lds_sync();  // each wave stages only its own 16 rows
 #pragma unroll
       for (int t = 0; t < KT; ++t) {
         const int jj = 16 * t + 4 * g;
@@ -156,7 +156,9 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
         if (pass == 0) x0[t] = make_float4(w[0], w[1], w[2], w[3]);
         else x1[t] = make_float4(w[0], w[1], w[2], w[3]);
       }
-      __syncthreads();
+      // pass 1: the whole block reuses the staging LDS as table buffers next
+      if (pass == 0) wave_lds_sync();
+      else __syncthreads();
     }
   } else {
     // Generic shapes: gather the fragments directly.
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
         os[(4 * g + i) * kOutPitch + 2 * c16] = a0[i];
         os[(4 * g + i) * kOutPitch + 2 * c16 + 1] = a1[i];
       }
-      __syncthreads();
+      wave_lds_sync();  // the staging tile is private to the wave
       const int col = lane & 31, l = lt + col;
 #pragma unroll
       for (int rr = 0; rr < 8; ++rr) {

@@ -79,6 +79,19 @@ struct dino_plan {
   int sched[4];
 };
 
+// Degree tiles of the row-resident Legendre kernels: 32-degree tiles that
+// cover l = m .. L-1, aligned so that the last one ends at L -- the top
+// degree then never sits alone in a tile -- unless the first would start
+// below degree 0 (then the first starts at 0 and the last runs past L).
+constexpr int kLegTile = 32;
+__host__ __device__ inline int leg_tile_start(int m, int L) {
+  const int s = L - kLegTile * ((L - m + kLegTile - 1) / kLegTile);
+  return s > 0 ? s : 0;
+}
+__host__ __device__ inline int leg_tile_count(int m, int L) {
+  return (L - leg_tile_start(m, L) + kLegTile - 1) / kLegTile;
+}
+
 // Orders this wave's LDS writes before its later LDS reads when the staging
 // region is private to the wave: no workgroup barrier, so the waves of a
 // block need not wait for each other.

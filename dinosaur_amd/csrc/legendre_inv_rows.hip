@@ -45,9 +45,8 @@ __device__ __forceinline__ void row_field(int m, int q, int* b, int* r) {
   }
 }
 
-__host__ __device__ inline int inv_tiles_of_m(int m, int L) {
-  return (L - (m / kTile) * kTile + kTile - 1) / kTile;
-}
+static_assert(kTile == kLegTile, "tile width shared with the modal kernels");
+__host__ __device__ inline int inv_tiles_of_m(int m, int L) { return leg_tile_count(m, L); }
 
 // float4 units per degree tile: NT node tiles x 2 classes x 64 lanes.
 __host__ __device__ constexpr int inv_tile_f4(int nt) { return 128 * nt; }
@@ -79,7 +78,7 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
   if (row0 >= nrows) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, c16 = lane & 15;
-  const int lt0 = (m / kTile) * kTile;
+  const int lt0 = leg_tile_start(m, L);
   const int ntile = inv_tiles_of_m(m, L);
   const float4* tb = itab + itile_off[m];
 
@@ -161,8 +160,8 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
     }
   }
 
-  // Epilogue: class E (even l - m) is the d-parity q = m & 1.
-  const int qe = m & 1;
+  // Epilogue: class E (even l - m) is the d-parity q = (lt0 - m) & 1.
+  const int qe = (lt0 - m) & 1;
   __syncthreads();  // every wave is done with the table buffers
   float* os = reinterpret_cast<float*>(lds) + wave * 16 * OPITCH;
   const int ndn = J - Jh;  // southern nodes (the equator of odd J is northern)
@@ -227,7 +226,7 @@ __global__ void inverse_fragments_kernel(int L, int Jh, int Jhp, int NT,
   const int tile = (int)(idx / per_tile);
   const int rem = (int)(idx - tile * per_tile);
   const int nt = rem / 128, q = (rem / 64) % 2, lane = rem % 64;
-  const int lt = (m / kTile) * kTile + kTile * tile;
+  const int lt = leg_tile_start(m, L) + kTile * tile;
   const int gg = lane / 16, jj = 16 * nt + lane % 16;
   const int64_t out = (itile_off[m] + idx) * 4;  // floats
   unsigned short h[4], lo[4];

@@ -17,8 +17,10 @@
 //
 // Reduction order inside a 16-node group t: lane group g = lane / 16 holds
 // nodes jj = 16 t + 4 g + e (e = 0..3), one float4 of the row per lane.  A
-// degree tile covers 32 consecutive degrees lt .. lt + 31 with lt a multiple
-// of 32; accumulator q (0 or 1) holds the degrees lt + 2 c + q, c = lane % 16.
+// degree tile covers 32 consecutive degrees lt .. lt + 31; the tiles of one m
+// end at the last degree (lt = leg_tile_start(m, L) + 32 k), so the zero
+// degrees l < m fill at most 31 slots of the first tile.  Accumulator q (0 or
+// 1) holds the degrees lt + 2 c + q, c = lane % 16.
 #include "dino_common.h"
 
 namespace dino {
@@ -41,9 +43,8 @@ __device__ __forceinline__ void row_field(int m, int q, int* b, int* r) {
   }
 }
 
-__host__ __device__ inline int tiles_of_m(int m, int L) {
-  return (L - (m / kTileDeg) * kTileDeg + kTileDeg - 1) / kTileDeg;
-}
+static_assert(kTileDeg == kLegTile, "tile width shared with the modal kernels");
+__host__ __device__ inline int tiles_of_m(int m, int L) { return leg_tile_count(m, L); }
 
 // float4 units per degree tile: 2 parities x 2 KS node groups x 64 lanes for
 // FP32, 2 parities x KS steps x (hi, lo) x 64 lanes for BF16 -- both 256 KS.
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
   if (row0 >= nrows) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, c16 = lane & 15;
-  const int lt0 = (m / kTileDeg) * kTileDeg;
+  const int lt0 = leg_tile_start(m, L);
   const int ntile = tiles_of_m(m, L);
   const float4* tb = ftab + ftile_off[m];
 
@@ -108,10 +109,10 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
 #pragma unroll
   for (int i = 0; i < PER; ++i) st[i] = ld4(tb + tid + 256 * i);
 
-  // Folded rows: X0 pairs with the even-d degrees of a tile (relative
-  // parity (m & 1)), X1 with the odd-d ones.  For odd m the roles of the
-  // sum and the difference swap.
-  const float sgn = (m & 1) ? -1.f : 1.f;
+  // Folded rows: X0 pairs with the even-d degrees of a tile (l - m of
+  // parity (lt0 - m) & 1), X1 with the odd-d ones.  When lt0 - m is odd the
+  // roles of the sum and the difference swap.
+  const float sgn = ((lt0 - m) & 1) ? -1.f : 1.f;
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
   float4 x0[KT], x1[KT];
   if constexpr (VIN) {
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
     }
   }
 
-  // Zero block: whole 32-degree blocks below m (left untouched on request).
+  // Zero block: the degrees below the first tile (left untouched on request).
   if (lt0 > 0 && !skip_zero) {
     if constexpr (VOUT) {
       const int n4 = lt0 / 4;
@@ -323,7 +324,7 @@ __global__ void forward_fragments_kernel(int L, int Jh, int Jhp, int ks,
   if (idx >= n) return;
   const int tile = (int)(idx / per_tile);
   const int rem = (int)(idx - tile * per_tile);
-  const int lt = (m / kTileDeg) * kTileDeg + kTileDeg * tile;
+  const int lt = leg_tile_start(m, L) + kTileDeg * tile;
   auto value = [&](int l, int jj) -> double {
     return (l >= m && l < L && jj < Jh) ? p64[m_off[m] + (int64_t)(l - m) * Jhp + jj] : 0.0;
   };

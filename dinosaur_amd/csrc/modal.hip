@@ -186,9 +186,9 @@ __global__ __launch_bounds__(256) void prep_kernel(const T* __restrict__ st, T* 
   const int64_t e = blockIdx.y;
   if (pos >= p.RL) return;
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);
-  // only the inverse Legendre transform reads this output, from degree
-  // 32 * floor(m / 32) on: the degrees below are not written
-  if (l < ((r + 1) >> 1) / 32 * 32) return;
+  // only the inverse Legendre transform reads this output, from the first
+  // degree of its tiles on: the degrees below are not written
+  if (l < leg_tile_start((r + 1) >> 1, p.L)) return;
   const MOff o = moff(p.R, p.L);
   Stencil<T> S(cst, o, p.R, p.L, r, l);
   const int K = p.K;
@@ -338,9 +338,9 @@ __global__ __launch_bounds__(256) void epilogue_adjoint_kernel(const T* __restri
   const int64_t e = blockIdx.y;
   if (pos >= p.RL) return;
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);
-  // only the inverse Legendre transform reads this output, from degree
-  // 32 * floor(m / 32) on: the degrees below are not written
-  if (l < ((r + 1) >> 1) / 32 * 32) return;
+  // only the inverse Legendre transform reads this output, from the first
+  // degree of its tiles on: the degrees below are not written
+  if (l < leg_tile_start((r + 1) >> 1, p.L)) return;
   const MOff o = moff(p.R, p.L);
   Stencil<T> S(cst, o, p.R, p.L, r, l);
   const int K = p.K;

@@ -37,7 +37,7 @@ int leg_inverse(const dino_plan* p, const void* in, void* out, int64_t B, int dt
   return legendre_inverse_mfma(p, in, out, B, prec, s);
 }
 
-// triangle_only: rows may skip the all-zero degrees below 32 * floor(m / 32)
+// triangle_only: rows may skip the all-zero degrees below leg_tile_start(m, L)
 // (the FP64 and tiled kernels write them anyway, which is also allowed).
 int leg_forward(const dino_plan* p, const void* in, void* out, int64_t B, int dtype, int prec,
                 hipStream_t s, bool triangle_only = false) {

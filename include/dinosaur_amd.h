@@ -46,12 +46,13 @@ enum {
 
 /* Flag or-ed into `precision` of dino_sht_forward, dino_sht_inverse_adjoint
  * and dino_legendre_forward: each modal row of zonal wavenumber m is written
- * only from degree 32 * floor(m / 32) on, and the entries below (all zero,
- * since P_l^m vanishes for l < m) are left untouched.  For callers whose
- * consumers never read them -- the inverse Legendre kernels start at that
- * degree, and the fused explicit-terms kernels treat l < m as zero -- it
- * saves writing about 44% of a TL255 modal array.  No reference
- * counterpart (XLA writes the dense einsum result). */
+ * only from degree s(m) = max(0, L - 32 * ceil((L - m) / 32)) on (the first
+ * degree of the 32-degree tiles that cover m .. L-1, aligned to end at L),
+ * and the entries below (all zero, since P_l^m vanishes for l < m) are left
+ * untouched.  For callers whose consumers never read them -- the inverse
+ * Legendre kernels start at s(m), and the fused explicit-terms kernels treat
+ * l < m as zero -- it saves writing about 45% of a TL255 modal array.  No
+ * reference counterpart (XLA writes the dense einsum result). */
 enum { DINO_SHT_TRIANGLE_ONLY = 0x100 };
 
 typedef struct dino_plan dino_plan;

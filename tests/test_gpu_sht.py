@@ -201,9 +201,9 @@ def test_tl255_round_trip_config1_all_modes(gpu_device):
 @pytest.mark.parametrize('fn', ['dino_sht_forward', 'dino_sht_inverse_adjoint'])
 def test_triangle_only_flag_leaves_zero_block_untouched(fn, gpu_device):
   """DINO_SHT_TRIANGLE_ONLY: each modal row of wavenumber m is written from
-  degree 32 * floor(m / 32) on, bitwise equal to the full call there, and the
-  degrees below keep what the buffer held (a NaN sentinel here).  TL127, FP32,
-  6 fields."""
+  degree s(m) = max(0, L - 32 ceil((L - m) / 32)) on, bitwise equal to the full
+  call there, and the degrees below keep what the buffer held (a NaN sentinel
+  here).  TL127, FP32, 6 fields."""
   import ctypes
   from dinosaur_amd import _lib
   print(f'[test] triangle-only flag {fn}', flush=True)
@@ -221,6 +221,8 @@ def test_triangle_only_flag_leaves_zero_block_untouched(fn, gpu_device):
   _lib.check(rc, fn)
   torch.cuda.synchronize()
   for r in range(plan.R):
-    lt0 = 32 * (((r + 1) // 2) // 32)
+    m = (r + 1) // 2
+    lt0 = max(0, plan.L - 32 * (-(-(plan.L - m) // 32)))
+    assert lt0 <= m
     assert torch.isnan(out[:, r, :lt0]).all(), r
     assert torch.equal(out[:, r, lt0:], full[:, r, lt0:]), r

@@ -68,6 +68,7 @@ struct dino_plan {
   int64_t* ftile_off;   // (M) first tile of each m, in 16-byte units
   float* ffrag32;       // FP32 fragments
   uint16_t* ffragbf;    // BF16 hi/lo fragments
+  float* sink;          // (256) store target of out-of-range lanes, never read
   // Inverse-transform tables in MFMA fragment order (legendre_inv_rows.hip).
   // int_nt = 16-node tiles the tables hold (1, 2, 4 or 8; 0 when Jh > 128).
   int int_nt;

@@ -92,6 +92,9 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
       const int idx = tid + 256 * i;
       st[i] = *reinterpret_cast<const floatx4*>(src + (idx < TF4 ? idx : 0));
     }
+    // Raw loads from clamped addresses; invalid entries are zeroed where the
+    // values are consumed, so no instruction here waits for the data and the
+    // loads stay in flight during the next tile's MFMAs.
     const int l = lt0 + kTile * tt + (lane & 31);
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
@@ -99,8 +102,7 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
       const bool ok = row < nrows && l < L;
       int b = 0, rr = 0;
       if (ok) row_field(m, row, &b, &rr);
-      const float v = modal[((int64_t)b * Rrows + rr) * L + (ok ? l : 0)];
-      ar[r] = ok ? v : 0.f;
+      ar[r] = modal[((int64_t)b * Rrows + rr) * L + (ok ? l : 0)];
     }
   };
   load_tile(0);
@@ -117,11 +119,20 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
       const int idx = tid + 256 * i;
       if (idx < TF4) buf[idx] = st[i];
     }
+    {
+      const bool lok = lt0 + kTile * tt + (lane & 31) < L;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) as[(2 * r + (lane >> 5)) * kAPitch + (lane & 31)] = ar[r];
+      for (int r = 0; r < 8; ++r) {
+        const bool ok = lok && row0 + wave * 16 + 2 * r + (lane >> 5) < nrows;
+        as[(2 * r + (lane >> 5)) * kAPitch + (lane & 31)] = ok ? ar[r] : 0.f;
+      }
+    }
     __syncthreads();
     // the last iteration reloads its own tile, which is never stored
     load_tile(tt + 1 < ntile ? tt + 1 : tt);
+    // keep the prefetch ahead of the MFMAs (the scheduler would otherwise
+    // sink the loads to the end of the tile, serialising them with it)
+    __builtin_amdgcn_sched_barrier(0);
     const floatx4 v0 = *reinterpret_cast<const floatx4*>(as + c16 * kAPitch + 8 * g);
     const floatx4 v1 = *reinterpret_cast<const floatx4*>(as + c16 * kAPitch + 8 * g + 4);
     const float v[8] = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};

@@ -81,8 +81,8 @@ __device__ __forceinline__ floatx4 ld4(const float4* p) {
 template <bool BF, int KS, bool VIN, bool VOUT>
 __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
     const float* __restrict__ fourier, float* __restrict__ modal,
-    const float4* __restrict__ ftab, const int64_t* __restrict__ ftile_off, int B, int L,
-    int J, int Jh, int Rrows, int mpair, int skip_zero) {
+    const float4* __restrict__ ftab, const int64_t* __restrict__ ftile_off,
+    float* __restrict__ sink, int B, int L, int J, int Jh, int Rrows, int mpair, int skip_zero) {
   constexpr int KT = 2 * KS;  // 16-node groups
   constexpr int TF4 = tile_f4(KS);
   constexpr int PER = TF4 / 256;
@@ -225,16 +225,24 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
     }
   }
 
-  // Output rows of this lane: 4 g + i of the wave's strip.
-  float* orow[4];
+  // Output rows of this lane as offsets from the block's first field (the
+  // block spans at most 64 fields, so they fit in 32 bits), -1 when out of
+  // range.  Direct stores: rows 4 g + i of the wave's strip; staged stores:
+  // rows 2 rr + lane / 32.  Out-of-range lanes store into `sink` instead of
+  // branching around the store, so every store instruction issues and the
+  // next tile's wait for its table loads can leave the stores in flight.
+  float* const obase = modal + (int64_t)(m == 0 ? row0 : row0 >> 1) * Rrows * L;
+  constexpr int NOR = VOUT ? 4 : 8;
+  int orow[NOR];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int row = row0 + wave * 16 + 4 * g + i;
-    orow[i] = nullptr;
+  for (int i = 0; i < NOR; ++i) {
+    const int rl = VOUT ? 4 * g + i : 2 * i + (lane >> 5);
+    const int row = row0 + wave * 16 + rl;
+    orow[i] = -1;
     if (row < nrows) {
       int b, r;
       row_field(m, row, &b, &r);
-      orow[i] = modal + ((int64_t)b * Rrows + r) * L;
+      orow[i] = ((b - (m == 0 ? row0 : row0 >> 1)) * Rrows + r) * L;
     }
   }
 
@@ -249,6 +257,9 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
 #pragma unroll
       for (int i = 0; i < PER; ++i) st[i] = ld4(nb + tid + 256 * i);
     }
+    // keep the prefetch ahead of the MFMAs (the scheduler would otherwise
+    // sink the loads to the end of the tile, serialising them with it)
+    __builtin_amdgcn_sched_barrier(0);
     floatx4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0;
     if constexpr (BF) {
 #pragma unroll
@@ -283,8 +294,10 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
     if constexpr (VOUT) {
       const int l = lt + 2 * c16;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        if (orow[i] && l < L) *reinterpret_cast<float2*>(orow[i] + l) = make_float2(a0[i], a1[i]);
+      for (int i = 0; i < 4; ++i) {
+        float* dst = (orow[i] >= 0 && l < L) ? obase + orow[i] + l : sink + 2 * lane;
+        *reinterpret_cast<float2*>(dst) = make_float2(a0[i], a1[i]);
+      }
     } else {
       // Unaligned rows: transpose the wave's 16 x 32 tile through LDS so that
       // each store instruction writes two contiguous 128-byte row segments.
@@ -299,12 +312,8 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
 #pragma unroll
       for (int rr = 0; rr < 8; ++rr) {
         const int rl = 2 * rr + (lane >> 5);
-        const int row = row0 + wave * 16 + rl;
-        if (row < nrows && l < L) {
-          int b, r;
-          row_field(m, row, &b, &r);
-          modal[((int64_t)b * Rrows + r) * L + l] = os[rl * kOutPitch + col];
-        }
+        float* dst = (orow[rr] >= 0 && l < L) ? obase + orow[rr] + l : sink + lane;
+        *dst = os[rl * kOutPitch + col];
       }
     }
   }
@@ -364,8 +373,9 @@ int launch_rows(const dino_plan* pl, const float* in, float* out, int64_t B, int
                          : reinterpret_cast<const float4*>(pl->ffrag32);
   const int mpair = (pl->sched[DINO_STAGE_LEGENDRE_FORWARD] & DINO_SCHED_MPAIRED) ? 1 : 0;
 #define ROWS_LAUNCH(VI, VO)                                                                  \
-  leg_fwd_rows_kernel<BF, KS, VI, VO><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, (int)B, \
-                                                            pl->L, pl->J, pl->Jh, pl->R, mpair, skip_zero)
+  leg_fwd_rows_kernel<BF, KS, VI, VO><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, pl->sink, \
+                                                            (int)B, pl->L, pl->J, pl->Jh, pl->R,  \
+                                                            mpair, skip_zero)
   if (vin && vout) ROWS_LAUNCH(true, true);
   else if (vin) ROWS_LAUNCH(true, false);
   else if (vout) ROWS_LAUNCH(false, true);
@@ -423,6 +433,9 @@ int legendre_forward_rows(const dino_plan* pl, const float* in, float* out, int6
                           bool split, bool triangle_only, hipStream_t s) {
   if (B == 0) return DINO_OK;
   DINO_REQUIRE(2 * B / kRowsPerBlock < (int64_t)1 << 30, DINO_EINVAL, "batch too large");
+  // a block's output rows span at most 64 fields, addressed with 32-bit offsets
+  DINO_REQUIRE((int64_t)kRowsPerBlock * pl->R * pl->L < (int64_t)1 << 31, DINO_EINVAL,
+               "modal fields too large for the row kernel");
   const int z = triangle_only ? 1 : 0;
   return split ? rows_by_ks<true>(pl, in, out, B, z, s) : rows_by_ks<false>(pl, in, out, B, z, s);
 }

@@ -184,7 +184,7 @@ static void free_plan(dino_plan* p) {
   void* bufs[] = {p->nodes64, p->weights64, p->qw32, p->qw64, p->m_offset,
                   p->p64, p->p32, p->p16_hi, p->p16_lo, p->pbf_hi, p->pbf_lo,
                   p->tw64, p->tw32, p->ftile_off, p->ffrag32, p->ffragbf,
-                  p->itile_off, p->ifrag32, p->ifragbf};
+                  p->itile_off, p->ifrag32, p->ifragbf, p->sink};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete p;
@@ -297,6 +297,7 @@ extern "C" int dino_plan_create(int M, int L, int I, int J, const double* nodes,
     PLAN_HIP(hipMalloc(&p->ftile_off, sizeof(int64_t) * M));
     PLAN_HIP(hipMalloc(&p->ffrag32, 16 * f4));
     PLAN_HIP(hipMalloc(&p->ffragbf, 16 * f4));
+    PLAN_HIP(hipMalloc(&p->sink, sizeof(float) * 256));
     PLAN_HIP(hipMemcpy(p->ftile_off, foff.data(), sizeof(int64_t) * M, hipMemcpyHostToDevice));
     forward_fragment_fill(p, 0);
     PLAN_HIP(hipGetLastError());

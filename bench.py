@@ -8,9 +8,12 @@ equations with specific humidity and two cloud tracers, 16 members, SIL3 with
 the exponential step filter at dt = 5 min, seeded synthetic initial states
 (rest atmosphere over seeded 2 km orography + seeded perturbations and
 humidity).  One benchmark step = one SIL3 step of the whole ensemble, forward
-and its adjoint (the rollout keeps a checkpoint per step and recomputes it in
-the backward pass).  The loss is quadratic in the final level-18 (sigma ~ 0.5)
-nodal temperature and vorticity.
+and its adjoint.  The rollout is cut into one-step checkpoint segments; a
+segment records its graph in the forward pass while HBM leaves room for it
+(no recomputation), the others keep only their input and are recomputed in
+the backward pass (--keep-graphs 0 recomputes every step; the JSON line also
+reports that rate).  The loss is quadratic in the final level-18 (sigma ~
+0.5) nodal temperature and vorticity.
 
 Legs, all in one run:
 * chosen   - the dinosaur_amd HIP pipeline (value of the JSON line);
@@ -49,6 +52,8 @@ def _parse():
   ap.add_argument('--members', type=int, default=MEMBERS)
   ap.add_argument('--precision', default='auto', help='float32 | bf16x3 | auto (= float32)')
   ap.add_argument('--plain-steps', type=int, default=1)
+  ap.add_argument('--keep-graphs', default='auto',
+                  help="rollout segments that keep their graphs: 'auto' (as HBM allows) or a count")
   ap.add_argument('--no-plain', action='store_true')
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--forward-only', action='store_true')
@@ -150,7 +155,8 @@ def build(torch, precision, members, impl=None, fused_nodal=True, dtype=None):
   return eq, state, coords, step, targets
 
 
-def rollout_fwd_adj(torch, step, state, coords, targets, steps, forward_only=False):
+def rollout_fwd_adj(torch, step, state, coords, targets, steps, forward_only=False,
+                    keep_graphs=0, report=None):
   from dinosaur_amd import primitive_equations as pe
   from dinosaur_amd import time_integration as ti
   grid = coords.horizontal
@@ -165,7 +171,8 @@ def rollout_fwd_adj(torch, step, state, coords, targets, steps, forward_only=Fal
       for _ in range(steps):
         y = step(y)
     return None
-  y = ti.checkpointed_rollout(step, s0, steps, checkpoint_every=1)
+  y = ti.checkpointed_rollout(step, s0, steps, checkpoint_every=1, keep_graphs=keep_graphs,
+                              report=report)
   t18 = grid.to_nodal(y.temperature_variation[..., 18, :, :])
   z18 = grid.to_nodal(y.vorticity[..., 18, :, :])
   loss = ((t18 - targets[0]) ** 2).mean() + 1e12 * ((z18 - targets[1]) ** 2).mean()
@@ -229,12 +236,14 @@ def accuracy(torch, precision, members=2, steps=1, impl=None, fused_nodal=True):
           'reference': 'same pipeline in FP64 (FP64 transforms, FP64 nodal terms)'}
 
 
-def time_leg(torch, step, state, coords, targets, steps, warmup, forward_only=False):
+def time_leg(torch, step, state, coords, targets, steps, warmup, forward_only=False,
+             keep_graphs=0, report=None):
   for _ in range(warmup):
-    rollout_fwd_adj(torch, step, state, coords, targets, 1, forward_only)
+    rollout_fwd_adj(torch, step, state, coords, targets, 1, forward_only, keep_graphs)
   _sync(torch)
   t0 = time.perf_counter()
-  out = rollout_fwd_adj(torch, step, state, coords, targets, steps, forward_only)
+  out = rollout_fwd_adj(torch, step, state, coords, targets, steps, forward_only, keep_graphs,
+                        report)
   _sync(torch)
   wall = time.perf_counter() - t0
   return wall, out
@@ -478,13 +487,28 @@ def main():
                                 'ms': {k: round(v, 3) for k, v in r['ms'].items()}}
                            for st, r in rep.items()} for b, rep in tuned.items()}
   print(f'[bench] autotune: {tune_summary}', file=sys.stderr, flush=True)
+  # Segments of one step; each records its graph in the forward pass while
+  # HBM leaves room (no recomputation), the rest are recomputed in the
+  # backward pass.  The all-recompute rate (the rate of rollouts too long
+  # for any graph to stay) is timed alongside.
+  keep = 'auto' if args.keep_graphs == 'auto' else int(args.keep_graphs)
+  rollout = {}
   wall, _ = time_leg(torch, step, state, coords, targets, args.steps, args.warmup,
-                     args.forward_only)
+                     args.forward_only, keep, rollout)
   days = DT_SECONDS / 86400.0
   value = members * args.steps * days / wall
   ms_per_step = wall / args.steps * 1e3
   print(f'[bench] chosen: {wall:.3f} s for {args.steps} steps, {ms_per_step:.1f} ms/step, '
-        f'{value:.3f} member-sim-days/s', file=sys.stderr, flush=True)
+        f'{value:.3f} member-sim-days/s, graphs kept for {rollout.get("kept")} of '
+        f'{rollout.get("segments")} steps', file=sys.stderr, flush=True)
+  torch.cuda.empty_cache()
+  recompute_ms = None
+  if not args.forward_only and rollout.get('kept'):
+    rwall, _ = time_leg(torch, step, state, coords, targets, args.steps, 0, False, 0)
+    recompute_ms = rwall / args.steps * 1e3
+    print(f'[bench] chosen, every step recomputed: {recompute_ms:.1f} ms/step',
+          file=sys.stderr, flush=True)
+    torch.cuda.empty_cache()
 
   # dominant kernel, timed live on its launch stream
   pcode = _lib.PRECISIONS[precision]
@@ -524,11 +548,12 @@ def main():
     print('[bench] plain FP32 leg (dense reference formulation)', file=sys.stderr, flush=True)
     _, pstate, pcoords, pstep, ptargets = build(torch, 'float32', members,
                                                 impl=DenseFP32Harmonics, fused_nodal=False)
+    prollout = {}
     pwall, _ = time_leg(torch, pstep, pstate, pcoords, ptargets, args.plain_steps, 1,
-                        args.forward_only)
+                        args.forward_only, keep, prollout)
     pval = members * args.plain_steps * days / pwall
     plain = {'value': pval, 'ms_per_step': pwall / args.plain_steps * 1e3,
-             'steps': args.plain_steps}
+             'steps': args.plain_steps, 'graphs_kept': prollout.get('kept')}
     print(f'[bench] plain: {pwall:.3f} s, {pval:.3f} member-sim-days/s', file=sys.stderr,
           flush=True)
     del pstate, pstep, ptargets
@@ -560,7 +585,13 @@ def main():
       'dtype': 'f32' if precision == 'float32' else f'f32 ({precision} Legendre)',
       'data': 'synthetic (seeded rest atmosphere + orography + perturbations + humidity)',
       'config': {'workload': f'{GRID} L{LAYERS} moist (q + 2 clouds), {members} members, '
-                             'SIL3 dt=5 min, forward + checkpointed adjoint',
+                             f'SIL3 dt=5 min, {args.steps}-step rollout, forward + adjoint '
+                             '(one-step checkpoint segments; graphs kept in HBM where '
+                             'they fit, the rest recomputed)',
+                 'rollout': {'steps': args.steps, 'graphs_kept': rollout.get('kept'),
+                             'recomputed': (rollout.get('segments') or 0) -
+                                           (rollout.get('kept') or 0),
+                             'all_recomputed_ms_per_step': recompute_ms},
                  'members': members, 'grid': GRID, 'levels': LAYERS,
                  'ensemble_sim_days_per_s': args.steps * days / wall,
                  'plain_fp32': plain,

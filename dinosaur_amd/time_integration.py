@@ -554,12 +554,23 @@ def _flatten_state(x):
   raise TypeError(f'cannot flatten {type(x)}')
 
 
-def checkpointed_rollout(step_fn, x, steps: int, checkpoint_every: int):
+def checkpointed_rollout(step_fn, x, steps: int, checkpoint_every: int, keep_graphs=0,
+                         report: dict | None = None):
   """Runs `steps` steps keeping only every `checkpoint_every`-th state for the
   backward pass; each segment is recomputed when gradients are taken (the
-  role of nested_checkpoint_scan, time_integration.py:573-637)."""
+  role of nested_checkpoint_scan, time_integration.py:573-637).
+
+  keep_graphs: segments that record their graphs in the forward pass instead
+  of being recomputed -- an int (the first `keep_graphs` segments) or 'auto'
+  (every segment for which free device memory still leaves room for two more
+  segment graphs and 4 GiB, the first segment's graph size estimated as
+  12 x its input bytes per step and later ones as the largest graph kept so
+  far).  Kept segments skip the recomputation; gradients are bitwise the
+  same.  `report`, if given, receives {'segments', 'kept'}."""
   if steps % checkpoint_every:
     raise ValueError(f'inconsistent steps={steps} and checkpoint_every={checkpoint_every}')
+  if not (keep_graphs == 'auto' or (isinstance(keep_graphs, int) and keep_graphs >= 0)):
+    raise ValueError(f"keep_graphs must be 'auto' or an int >= 0, got {keep_graphs!r}")
   as_flat = getattr(x, 'as_flat', None)
   if as_flat is not None:
     x = as_flat()
@@ -575,14 +586,41 @@ def checkpointed_rollout(step_fn, x, steps: int, checkpoint_every: int):
     out, structure['rebuild'] = _flatten_state(y)
     return tuple(out)
 
-  for _ in range(steps // checkpoint_every):
+  n_seg = steps // checkpoint_every
+  kept, graph_bytes = 0, None
+  for _ in range(n_seg):
     fn = functools.partial(segment, rebuild)
     if torch.is_grad_enabled() and any(t.requires_grad for t in leaves):
-      leaves = list(_CheckpointSegment.apply(fn, *leaves))
+      if _keep_segment(keep_graphs, kept, graph_bytes, leaves, checkpoint_every):
+        dev = leaves[0].device
+        before = torch.cuda.memory_allocated(dev) if dev.type == 'cuda' else 0
+        leaves = list(fn(*leaves))
+        if dev.type == 'cuda':
+          graph_bytes = max(graph_bytes or 0, torch.cuda.memory_allocated(dev) - before)
+        kept += 1
+      else:
+        leaves = list(_CheckpointSegment.apply(fn, *leaves))
     else:
       leaves = list(fn(*leaves))
     rebuild = structure['rebuild']
+  if report is not None:
+    report.update(segments=n_seg, kept=kept)
   return rebuild(leaves)
+
+
+def _keep_segment(policy, kept, graph_bytes, leaves, steps_per_segment) -> bool:
+  """Whether the next segment records its graph in the forward pass."""
+  if policy != 'auto':
+    return kept < policy
+  dev = leaves[0].device
+  if dev.type != 'cuda':
+    return False
+  need = graph_bytes
+  if need is None:
+    need = 12 * steps_per_segment * sum(t.numel() * t.element_size() for t in leaves)
+  free, _ = torch.cuda.mem_get_info(dev)
+  free += torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev)
+  return free - need >= 2 * need + (4 << 30)
 
 
 class _CheckpointSegment(torch.autograd.Function):

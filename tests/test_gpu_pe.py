@@ -329,11 +329,13 @@ def test_fused_imex_stage_matches_elementwise_passes(gpu_device):
     assert torch.allclose(x, y, rtol=1e-6, atol=0)
 
 
-def test_checkpointed_rollout_gradient_matches_plain_autograd(gpu_device):
-  """checkpointed_rollout (segments recomputed in the backward pass) gives the
-  same final state and gradient as plain autograd through the same steps,
-  starting from a leaf-based state (FP64, T21, 3 levels)."""
-  print('[test] checkpointed rollout gradient', flush=True)
+@pytest.mark.parametrize('keep', [0, 1, 'auto'])
+def test_checkpointed_rollout_gradient_matches_plain_autograd(keep, gpu_device):
+  """checkpointed_rollout (segments recomputed in the backward pass, or --
+  keep -- recorded in the forward pass while memory allows) gives the same
+  final state and gradient as plain autograd through the same steps, starting
+  from a leaf-based state (FP64, T21, 3 levels)."""
+  print(f'[test] checkpointed rollout gradient keep={keep}', flush=True)
   grid = sh.Grid.T21()
   K = 3
   coords = coordinate_systems.CoordinateSystem(grid, sigma_coordinates.SigmaCoordinates.equidistant(K))
@@ -348,7 +350,9 @@ def test_checkpointed_rollout_gradient_matches_plain_autograd(gpu_device):
     tr = {k: v.clone().requires_grad_(True) for k, v in st.tracers.items()}
     s0 = pe.State(**leaves, tracers=tr)
     if checkpoint:
-      y = ti.checkpointed_rollout(step, s0, 4, checkpoint_every=2)
+      rep = {}
+      y = ti.checkpointed_rollout(step, s0, 4, checkpoint_every=2, keep_graphs=keep, report=rep)
+      assert rep == {'segments': 2, 'kept': 2 if keep == 'auto' else keep}
     else:
       y = s0
       for _ in range(4):

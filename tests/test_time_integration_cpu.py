@@ -2,6 +2,7 @@
 any torch tensors; the reference's own known answers,
 time_integration_test.py:303-335)."""
 import numpy as np
+import pytest
 import torch
 
 from dinosaur_amd import time_integration as ti
@@ -50,3 +51,29 @@ def test_maybe_fix_sim_time_roundoff():
     sim_time = 0.30000000000000004
   s = ti.maybe_fix_sim_time_roundoff(S(), 0.1)
   assert abs(s.sim_time - 0.3) < 1e-15
+
+
+def test_checkpointed_rollout_keep_graphs_policy():
+  """Segments that record their graphs in the forward pass (keep_graphs) give
+  bitwise the same state and gradient as recomputed segments and as plain
+  autograd; the report counts them; 'auto' keeps none off the GPU; invalid
+  policies are rejected."""
+  torch.manual_seed(0)
+  w = torch.randn(8, 8, dtype=torch.float64) * 0.3
+  step = lambda y: torch.tanh(y @ w) + 0.1 * y  # noqa: E731
+  x0 = torch.randn(4, 8, dtype=torch.float64)
+  xa = x0.clone().requires_grad_(True)
+  y_ref = xa
+  for _ in range(6):
+    y_ref = step(y_ref)
+  g_ref, = torch.autograd.grad((y_ref ** 2).sum(), xa)
+  for keep, kept in ((0, 0), (2, 2), (5, 3), ('auto', 0)):
+    x = x0.clone().requires_grad_(True)
+    rep = {}
+    y = ti.checkpointed_rollout(step, x, 6, checkpoint_every=2, keep_graphs=keep, report=rep)
+    g, = torch.autograd.grad((y ** 2).sum(), x)
+    assert rep == {'segments': 3, 'kept': kept}, (keep, rep)
+    assert torch.equal(y.detach(), y_ref.detach()) and torch.equal(g, g_ref), keep
+  for bad in (-1, 'all', 1.5):
+    with pytest.raises(ValueError):
+      ti.checkpointed_rollout(step, x0.clone().requires_grad_(True), 6, 2, keep_graphs=bad)

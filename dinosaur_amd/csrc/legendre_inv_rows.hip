@@ -60,36 +60,50 @@ __device__ __forceinline__ float bf16_value(unsigned short h) {
 }
 
 // VOUT: J % 8 == 0 and a 16-byte aligned Fourier buffer (float4 stores).
-template <bool BF, int NT, bool VOUT>
-__global__ __launch_bounds__(256) void leg_inv_rows_kernel(
+// WS: waves per 16-row strip (1 or 2).  With 2 the two waves of a strip split
+// the node tiles, the modal loads and the output stores between them, so a
+// block has 8 waves that each hold half the accumulators: twice the waves
+// per CU to overlap the MFMAs with the loads and stores of other waves (the
+// register budget of 6 waves per SIMD lets 3 such blocks share a CU).
+template <bool BF, int NT, bool VOUT, int WS>
+__global__ __launch_bounds__(256 * WS, WS == 2 ? 6 : 1) void leg_inv_rows_kernel(
     const float* __restrict__ modal, float* __restrict__ fourier,
     const float4* __restrict__ itab, const int64_t* __restrict__ itile_off, int B, int L,
     int J, int Jh, int Rrows, int mpair) {
+  static_assert(NT % WS == 0, "the waves of a strip split the node tiles evenly");
+  constexpr int NTH = 256 * WS;  // threads
+  constexpr int NTW = NT / WS;   // node tiles per wave
+  constexpr int RPW = 8 / WS;    // modal values per lane per tile
   constexpr int TF4 = inv_tile_f4(NT);
-  constexpr int PER = (TF4 + 255) / 256;
+  constexpr int PER = (TF4 + NTH - 1) / NTH;
   constexpr int OPITCH = 16 * NT + 4;
   constexpr int OUT4 = 4 * 16 * OPITCH / 4;
   constexpr int MAIN4 = (2 * TF4 > OUT4) ? 2 * TF4 : OUT4;
+  // A staging: one buffer per strip, double-buffered when two waves share it
+  // (one may write the next tile while the other still reads this one)
+  constexpr int NAS = WS == 1 ? 1 : 2;
   __shared__ floatx4 lds[MAIN4];
-  __shared__ float astg[4 * 16 * kAPitch];
+  __shared__ float astg[NAS * 4 * 16 * kAPitch];
   const int m = m_of_block(blockIdx.y, gridDim.y, mpair);
   const int nrows = (m == 0) ? B : 2 * B;
   const int row0 = blockIdx.x * kRows;
   if (row0 >= nrows) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int strip = wave & 3, wh = wave >> 2;  // wh: this wave's share of the strip
   const int g = lane >> 4, c16 = lane & 15;
   const int lt0 = leg_tile_start(m, L);
   const int ntile = inv_tiles_of_m(m, L);
   const float4* tb = itab + itile_off[m];
 
-  // prefetch registers: table tile and the wave's 16 x 32 modal segment
+  // prefetch registers: table tile and this wave's rows of the strip's 16 x 32
+  // modal segment (rows 2 rr + lane / 32, rr = r WS + wh)
   floatx4 st[PER];
-  float ar[8];
+  float ar[RPW];
   auto load_tile = [&](int tt) {
     const float4* src = tb + (int64_t)tt * TF4;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int idx = tid + 256 * i;
+      const int idx = tid + NTH * i;
       st[i] = *reinterpret_cast<const floatx4*>(src + (idx < TF4 ? idx : 0));
     }
     // Raw loads from clamped addresses; invalid entries are zeroed where the
@@ -97,8 +111,8 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
     // loads stay in flight during the next tile's MFMAs.
     const int l = lt0 + kTile * tt + (lane & 31);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int row = row0 + wave * 16 + 2 * r + (lane >> 5);
+    for (int r = 0; r < RPW; ++r) {
+      const int row = row0 + strip * 16 + 2 * (r * WS + wh) + (lane >> 5);
       const bool ok = row < nrows && l < L;
       int b = 0, rr = 0;
       if (ok) row_field(m, row, &b, &rr);
@@ -107,24 +121,25 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
   };
   load_tile(0);
 
-  floatx4 acc[NT][2];
+  floatx4 acc[NTW][2];
 #pragma unroll
-  for (int n = 0; n < NT; ++n) acc[n][0] = acc[n][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int n = 0; n < NTW; ++n) acc[n][0] = acc[n][1] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  float* as = astg + wave * 16 * kAPitch;
   for (int tt = 0; tt < ntile; ++tt) {
     floatx4* buf = lds + (tt & 1) * TF4;
+    float* as = astg + ((NAS == 2 ? (tt & 1) : 0) * 4 + strip) * 16 * kAPitch;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int idx = tid + 256 * i;
+      const int idx = tid + NTH * i;
       if (idx < TF4) buf[idx] = st[i];
     }
     {
       const bool lok = lt0 + kTile * tt + (lane & 31) < L;
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const bool ok = lok && row0 + wave * 16 + 2 * r + (lane >> 5) < nrows;
-        as[(2 * r + (lane >> 5)) * kAPitch + (lane & 31)] = ok ? ar[r] : 0.f;
+      for (int r = 0; r < RPW; ++r) {
+        const int rl = 2 * (r * WS + wh) + (lane >> 5);
+        const bool ok = lok && row0 + strip * 16 + rl < nrows;
+        as[rl * kAPitch + (lane & 31)] = ok ? ar[r] : 0.f;
       }
     }
     __syncthreads();
@@ -148,10 +163,10 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
           lo[q][e] = (short)bf16_bits(x - bf16_value(h));
         }
 #pragma unroll
-      for (int n = 0; n < NT; ++n)
+      for (int n = 0; n < NTW; ++n)
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-          const floatx4 p = buf[(n * 2 + q) * 64 + lane];
+          const floatx4 p = buf[((wh * NTW + n) * 2 + q) * 64 + lane];
           const shortx4 ph = __builtin_bit_cast(shortx4, __builtin_shufflevector(p, p, 0, 1));
           const shortx4 pl = __builtin_bit_cast(shortx4, __builtin_shufflevector(p, p, 2, 3));
           acc[n][q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lo[q], ph, acc[n][q], 0, 0, 0);
@@ -160,10 +175,10 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
         }
     } else {
 #pragma unroll
-      for (int n = 0; n < NT; ++n)
+      for (int n = 0; n < NTW; ++n)
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-          const floatx4 p = buf[(n * 2 + q) * 64 + lane];
+          const floatx4 p = buf[((wh * NTW + n) * 2 + q) * 64 + lane];
 #pragma unroll
           for (int u = 0; u < 4; ++u)
             acc[n][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[2 * u + q], p[u], acc[n][q], 0, 0, 0);
@@ -174,16 +189,21 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
   // Epilogue: class E (even l - m) is the d-parity q = (lt0 - m) & 1.
   const int qe = (lt0 - m) & 1;
   __syncthreads();  // every wave is done with the table buffers
-  float* os = reinterpret_cast<float*>(lds) + wave * 16 * OPITCH;
+  float* os = reinterpret_cast<float*>(lds) + strip * 16 * OPITCH;
   const int ndn = J - Jh;  // southern nodes (the equator of odd J is northern)
+  // the output staging rows belong to one strip: its own wave, or two waves
+  auto strip_sync = [&]() {
+    if constexpr (WS == 1) wave_lds_sync();
+    else __syncthreads();
+  };
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
     const int len = half == 0 ? Jh : ndn;
 #pragma unroll
-    for (int n = 0; n < NT; ++n)
+    for (int n = 0; n < NTW; ++n)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int jj = 16 * n + c16;
+        const int jj = 16 * (wh * NTW + n) + c16;
         // selects, not a runtime array index (which would demote acc to scratch)
         const float a0 = acc[n][0][i], a1 = acc[n][1][i];
         const float e = qe ? a1 : a0, o = qe ? a0 : a1;
@@ -194,13 +214,13 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
           if (jj < Jh && pos < ndn) os[(4 * g + i) * OPITCH + pos] = e - o;
         }
       }
-    wave_lds_sync();  // the output staging rows are private to the wave
+    strip_sync();
     const int col0 = half == 0 ? J - Jh : 0;
     if constexpr (VOUT) {
       const int n4 = len / 4;  // J % 8 == 0: both halves hold Jh floats
-      for (int idx = lane; idx < 16 * n4; idx += 64) {
+      for (int idx = lane + 64 * wh; idx < 16 * n4; idx += 64 * WS) {
         const int rl = idx / n4, c = idx - rl * n4;
-        const int row = row0 + wave * 16 + rl;
+        const int row = row0 + strip * 16 + rl;
         if (row < nrows) {
           int b, r;
           row_field(m, row, &b, &r);
@@ -209,9 +229,9 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
         }
       }
     } else {
-      for (int idx = lane; idx < 16 * len; idx += 64) {
+      for (int idx = lane + 64 * wh; idx < 16 * len; idx += 64 * WS) {
         const int rl = idx / len, c = idx - rl * len;
-        const int row = row0 + wave * 16 + rl;
+        const int row = row0 + strip * 16 + rl;
         if (row < nrows) {
           int b, r;
           row_field(m, row, &b, &r);
@@ -219,7 +239,7 @@ __global__ __launch_bounds__(256) void leg_inv_rows_kernel(
         }
       }
     }
-    wave_lds_sync();
+    strip_sync();
   }
 }
 
@@ -265,12 +285,14 @@ int launch_inv_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
   const float4* tab = BF ? reinterpret_cast<const float4*>(pl->ifragbf)
                          : reinterpret_cast<const float4*>(pl->ifrag32);
   const int mpair = (pl->sched[DINO_STAGE_LEGENDRE_INVERSE] & DINO_SCHED_MPAIRED) ? 1 : 0;
+  // two waves per strip whenever the node tiles split evenly
+  constexpr int WS = NT % 2 == 0 ? 2 : 1;
   if (vout)
-    leg_inv_rows_kernel<BF, NT, true><<<grid, 256, 0, s>>>(in, out, tab, pl->itile_off, (int)B,
-                                                           pl->L, pl->J, pl->Jh, pl->R, mpair);
+    leg_inv_rows_kernel<BF, NT, true, WS><<<grid, 256 * WS, 0, s>>>(
+        in, out, tab, pl->itile_off, (int)B, pl->L, pl->J, pl->Jh, pl->R, mpair);
   else
-    leg_inv_rows_kernel<BF, NT, false><<<grid, 256, 0, s>>>(in, out, tab, pl->itile_off, (int)B,
-                                                            pl->L, pl->J, pl->Jh, pl->R, mpair);
+    leg_inv_rows_kernel<BF, NT, false, WS><<<grid, 256 * WS, 0, s>>>(
+        in, out, tab, pl->itile_off, (int)B, pl->L, pl->J, pl->Jh, pl->R, mpair);
   DINO_LAUNCH_CHECK("legendre inverse rows kernel");
   return DINO_OK;
 }

@@ -64,9 +64,10 @@ __device__ __forceinline__ float bf16_value(unsigned short h) {
 // the node tiles, the modal loads and the output stores between them, so a
 // block has 8 waves that each hold half the accumulators: twice the waves
 // per CU to overlap the MFMAs with the loads and stores of other waves (the
-// register budget of 6 waves per SIMD lets 3 such blocks share a CU).
+// FP32 register budget of 6 waves per SIMD lets 3 such blocks share a CU;
+// the split path would spill under it).
 template <bool BF, int NT, bool VOUT, int WS>
-__global__ __launch_bounds__(256 * WS, WS == 2 ? 6 : 1) void leg_inv_rows_kernel(
+__global__ __launch_bounds__(256 * WS, WS == 2 && !BF ? 6 : 1) void leg_inv_rows_kernel(
     const float* __restrict__ modal, float* __restrict__ fourier,
     const float4* __restrict__ itab, const int64_t* __restrict__ itile_off, int B, int L,
     int J, int Jh, int Rrows, int mpair) {

@@ -72,6 +72,8 @@ SIGNATURES = {
                                       _c_i64, _c_i64, _c_i64, _c_i64, _c_i, _c_i, _c_p]),
     'dino_imex_stage_adjoint_targets': (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p, _c_d, _c_i64, _c_i64,
                                                _c_i64, _c_i64, _c_i64, _c_i, _c_i, _c_p]),
+    'dino_imex_adjoint_gather': (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_d, _c_p, _c_i64, _c_i, _c_i64,
+                                        _c_i64, _c_i64, _c_i64, _c_i, _c_i, _c_p]),
     'dino_implicit_terms_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i,
                                            _c_p]),
     'dino_nodal_create': (_c_i, [_c_i, _c_i, _c_i, _c_i, _c_i, ctypes.POINTER(_c_i), _c_i, _c_i,

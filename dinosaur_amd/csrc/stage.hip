@@ -243,6 +243,68 @@ __global__ __launch_bounds__(256) void targets_kernel(TargetArgs<T> a, int64_t n
   }
 }
 
+// Gather: one cotangent from several sources, out = v_0 + v_1 + ... in source
+// order, v_k = src_k (identity) or c_k (dt src_k) -- the same roundings and
+// order as a first targets pass writing v_0 followed by accumulating passes.
+// Output element o is state element i = o + off (off = g_origin for a ranged
+// output, else 0); a ranged output is written only where i is in [ib, ie).
+constexpr int kMaxSources = 8;
+
+template <typename T>
+struct GatherArgs {
+  const T* src[kMaxSources];
+  T c[kMaxSources];
+  int ident[kMaxSources];
+  int n;
+  T dt;
+  int64_t off, ib, ie;
+  int64_t trl;
+  int tl;
+};
+
+template <typename T, typename U>
+__device__ __forceinline__ U gather_sum(const GatherArgs<T>& a, int64_t idx) {
+  auto term = [&](int k) {
+    const U g = reinterpret_cast<const U*>(a.src[k])[idx];
+    return a.ident[k] ? g : a.c[k] * (a.dt * g);
+  };
+  U s = term(0);
+#pragma unroll
+  for (int k = 1; k < kMaxSources; ++k)
+    if (k < a.n) s = s + term(k);
+  return s;
+}
+
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void gather_kernel(GatherArgs<T> a, T* __restrict__ out,
+                                                     int64_t n_out) {
+  typedef typename Vec<T>::type V;
+  constexpr int W = VEC ? Vec<T>::W : 1;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nv = n_out / W;
+  auto one = [&](int64_t o) {
+    const int64_t i = o + a.off;
+    if (i < a.ib || i >= a.ie) return;
+    out[o] = below_triangle(i, 1, a.trl, a.tl) ? (T)0 : gather_sum<T, T>(a, i);
+  };
+  if (k < nv) {
+    if constexpr (VEC) {
+      const int64_t o0 = k * W, i0 = o0 + a.off;
+      if (i0 >= a.ib && i0 + W <= a.ie) {
+        reinterpret_cast<V*>(out)[k] =
+            below_triangle(i0, W, a.trl, a.tl) ? V{} : gather_sum<T, V>(a, i0 / W);
+      } else {
+#pragma unroll
+        for (int e = 0; e < W; ++e) one(o0 + e);
+      }
+    } else {
+      one(k);
+    }
+  } else if (VEC && k - nv < n_out - nv * W) {
+    one(nv * W + (k - nv));
+  }
+}
+
 inline bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
 
 template <typename T>
@@ -341,6 +403,36 @@ int launch_targets(const void* gbar, int nt, void* const* outp, const double* co
   else
     targets_kernel<T, false><<<blocks, 256, 0, s>>>(args, n);
   DINO_LAUNCH_CHECK("imex stage targets kernel");
+  return DINO_OK;
+}
+
+template <typename T>
+int launch_gather(const void* const* src, const double* coef, const int* ident, int ns,
+                  double dt, void* out, int64_t n_out, int64_t off, int64_t ib, int64_t ie,
+                  hipStream_t s, int64_t trl, int tl) {
+  GatherArgs<T> args{};
+  args.n = ns;
+  args.dt = (T)dt;
+  args.off = off;
+  args.ib = ib;
+  args.ie = ie;
+  args.trl = trl;
+  args.tl = tl;
+  bool vec = aligned16(out) && off % Vec<T>::W == 0;
+  for (int k = 0; k < ns; ++k) {
+    args.src[k] = static_cast<const T*>(src[k]);
+    args.c[k] = (T)coef[k];
+    args.ident[k] = ident[k] ? 1 : 0;
+    vec = vec && aligned16(src[k]);
+  }
+  const int W = vec ? Vec<T>::W : 1;
+  const int64_t threads = n_out / W + (n_out % W);
+  const unsigned blocks = (unsigned)((threads + 255) / 256);
+  if (vec)
+    gather_kernel<T, true><<<blocks, 256, 0, s>>>(args, static_cast<T*>(out), n_out);
+  else
+    gather_kernel<T, false><<<blocks, 256, 0, s>>>(args, static_cast<T*>(out), n_out);
+  DINO_LAUNCH_CHECK("imex gather kernel");
   return DINO_OK;
 }
 
@@ -445,6 +537,41 @@ extern "C" int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
                                   degrees);
   return launch_targets<float>(gbar, n_targets, out, coef, kind, dt, n, im_begin, im_end,
                                g_origin, as_stream(stream), degrees ? field_size : 0, degrees);
+}
+
+extern "C" int dino_imex_adjoint_gather(const void* const* src, const double* coef,
+                                        const int* identity, int n_src, double dt, void* out,
+                                        int64_t n, int ranged, int64_t im_begin,
+                                        int64_t im_end, int64_t g_origin, int64_t field_size,
+                                        int degrees, int dtype, void* stream) {
+  const char* name = "dino_imex_adjoint_gather";
+  DINO_REQUIRE(degrees == 0 || (degrees > 0 && field_size > 0 && field_size % degrees == 0 &&
+                                (field_size / degrees) % 2 == 1 && n % field_size == 0),
+               DINO_EINVAL, "%s: invalid Real-layout field shape (%lld, %d)", name,
+               (long long)field_size, degrees);
+  DINO_REQUIRE(n_src >= 1 && n_src <= kMaxSources, DINO_EINVAL,
+               "%s: need 1 to %d sources (got %d)", name, kMaxSources, n_src);
+  DINO_REQUIRE(src && coef && identity && out, DINO_EINVAL, "%s: NULL argument", name);
+  DINO_REQUIRE(n >= 0, DINO_EINVAL, "%s: negative size", name);
+  DINO_REQUIRE(dtype == DINO_F32 || dtype == DINO_F64, DINO_EINVAL, "%s: invalid dtype %d",
+               name, dtype);
+  for (int k = 0; k < n_src; ++k) DINO_REQUIRE(src[k], DINO_EINVAL, "%s: NULL source %d", name, k);
+  int64_t ib = 0, ie = n, off = 0, n_out = n;
+  if (ranged) {
+    int rc = check_range(name, n, im_begin, im_end, g_origin);
+    if (rc != DINO_OK) return rc;
+    ib = im_begin;
+    ie = im_end;
+    off = g_origin;
+    n_out = im_end - g_origin;
+  }
+  if (n_out == 0) return DINO_OK;
+  const int64_t trl = degrees ? field_size : 0;
+  if (dtype == DINO_F64)
+    return launch_gather<double>(src, coef, identity, n_src, dt, out, n_out, off, ib, ie,
+                                 as_stream(stream), trl, degrees);
+  return launch_gather<float>(src, coef, identity, n_src, dt, out, n_out, off, ib, ie,
+                              as_stream(stream), trl, degrees);
 }
 
 extern "C" int dino_imex_stage_adjoint(const void* out_cot, int n_ex, const double* a,

@@ -227,18 +227,25 @@ def _fused_stage(ops, y0, dt, ex, im):
   return out
 
 
-def _fused_targets(ops, gbar, dt, targets):
-  """One pass over gbar writing or accumulating every (buffer, coef, kind)."""
+def _fused_gather(ops, sources, dt, like, ranged=False):
+  """One cotangent from its pending (source, coef, identity) contributions,
+  summed in list order in one pass (at most 8 sources: _fused_imex_step takes
+  tableaus of at most 7 stages); None when there are none."""
   from . import _lib  # pylint: disable=g-import-not-at-top
-  if not targets:
-    return
-  stream = ctypes.c_void_p(torch.cuda.current_stream(gbar.device).cuda_stream)
-  kinds = (ctypes.c_int * len(targets))(*[k for _, _, k in targets])
-  rc = _lib.lib().dino_imex_stage_adjoint_targets(
-      ctypes.c_void_p(gbar.data_ptr()), len(targets), _ptrs([b for b, _, _ in targets]),
-      _dbls([c for _, c, _ in targets]), kinds, float(dt), gbar.numel(), ops.ib, ops.ie,
-      ops.go, ops.field_size, ops.degrees, _code(gbar), stream)
-  _lib.check(rc, 'dino_imex_stage_adjoint_targets')
+  if not sources:
+    return None
+  if len(sources) > 8:
+    raise ValueError(f'{len(sources)} cotangent sources; at most 8 are gathered')
+  out = ops.new_compact(like) if ranged else torch.empty_like(like)
+  stream = ctypes.c_void_p(torch.cuda.current_stream(like.device).cuda_stream)
+  ident = (ctypes.c_int * len(sources))(*[1 if i else 0 for _, _, i in sources])
+  rc = _lib.lib().dino_imex_adjoint_gather(
+      _ptrs([t for t, _, _ in sources]), _dbls([c for _, c, _ in sources]), ident,
+      len(sources), float(dt), ctypes.c_void_p(out.data_ptr()), like.numel(),
+      1 if ranged else 0, ops.ib, ops.ie, ops.go, ops.field_size, ops.degrees, _code(like),
+      stream)
+  _lib.check(rc, 'dino_imex_adjoint_gather')
+  return out
 
 
 def _fused_forward(ops, tab, dt, y0, keep):
@@ -279,9 +286,11 @@ class _FusedImexStep(torch.autograd.Function):
   The forward keeps, for each explicit evaluation, its autograd graph; the
   backward walks the stages in reverse: explicit adjoints through those
   graphs, implicit-term adjoints accumulated in place, the solve adjoint in
-  place, and every stage combination's cotangents written or accumulated
-  in one pass (dino_imex_stage_adjoint_targets).  The result equals
-  autograd through the same stages up to FP rounding of the sums.
+  place, and each term's cotangent formed once, when its stage is reached,
+  from all its contributions in one pass (dino_imex_adjoint_gather) instead
+  of being written at the output and updated at every later stage.  The
+  result equals autograd through the same stages up to FP rounding of the
+  sums.
   """
 
   @staticmethod
@@ -294,66 +303,59 @@ class _FusedImexStep(torch.autograd.Function):
 
   @staticmethod
   def backward(ctx, obar):
-    from . import _lib  # pylint: disable=g-import-not-at-top
     ops, tab, dt, f, leaves = ctx.ops, ctx.tab, ctx.dt, ctx.f, ctx.leaves
     a_ex, a_im, b_ex, b_im = tab.a_ex, tab.a_im, tab.b_ex, tab.b_im
-    ACC, IDN, RNG = _lib.TARGET_ACCUMULATE, _lib.TARGET_IDENTITY, _lib.TARGET_RANGED
     obar = obar.contiguous()
     n = len(b_ex)
-    fbar, gbar = [None] * n, [None] * n
-
-    def targets_for(coef_ex, coef_im, first_y0):
-      out = [(ybar0, 1.0, IDN if first_y0 else IDN | ACC)]
-      for j, c in coef_ex:
-        if fbar[j] is None:
-          fbar[j] = torch.empty_like(obar)
-          out.append((fbar[j], c, 0))
-        else:
-          out.append((fbar[j], c, ACC))
-      for j, c in coef_im:
-        if gbar[j] is None:
-          gbar[j] = ops.new_compact(obar)
-          out.append((gbar[j], c, RNG))
-        else:
-          out.append((gbar[j], c, RNG | ACC))
-      return out
-
-    ybar0 = torch.empty_like(obar)
-    _fused_targets(ops, obar, dt, targets_for(
-        [(j, b_ex[j]) for j in range(n) if b_ex[j] and f[j] is not None],
-        [(j, b_im[j]) for j in range(n) if b_im[j] and ctx.has_g[j]], True))
+    # Each cotangent is formed once, when it is first needed, from the list of
+    # its contributions (output cotangent, then the stage cotangents in
+    # reverse stage order): the same sums, in the same order, as writing it
+    # at the output combination and accumulating at every stage.
+    pend_f = {j: [(obar, b_ex[j], False)] for j in range(n) if b_ex[j] and f[j] is not None}
+    pend_g = {j: [(obar, b_im[j], False)] for j in range(n) if b_im[j] and ctx.has_g[j]}
+    pend_y0 = [(obar, 1.0, True)]
     for i in range(n - 1, 0, -1):
+      fbar = _fused_gather(ops, pend_f.pop(i, None), dt, obar)
+      gbar = _fused_gather(ops, pend_g.pop(i, None), dt, obar, ranged=True)
       ybar = None
-      if f[i] is not None and fbar[i] is not None:
-        (ybar,) = torch.autograd.grad(f[i], leaves[i], fbar[i])
-      if ctx.has_g[i] and gbar[i] is not None:
+      if f[i] is not None and fbar is not None:
+        (ybar,) = torch.autograd.grad(f[i], leaves[i], fbar)
+      if ctx.has_g[i] and gbar is not None:
         if ybar is None:
           ybar = torch.zeros_like(obar)
-        ops.terms_adjoint_acc(gbar[i], ybar)
-      f[i] = leaves[i] = fbar[i] = gbar[i] = None
+        ops.terms_adjoint_acc(gbar, ybar)
+      f[i] = leaves[i] = None
+      del fbar, gbar
       if ybar is None:
         continue
       ybar = ybar.contiguous()
       ops.solve_adjoint_(ybar, dt * a_im[i - 1][i])
-      _fused_targets(ops, ybar, dt, targets_for(
-          [(j, a_ex[i - 1][j]) for j in range(i) if a_ex[i - 1][j] and f[j] is not None],
-          [(j, a_im[i - 1][j]) for j in range(i) if a_im[i - 1][j] and ctx.has_g[j]], False))
-    if f[0] is not None and fbar[0] is not None:
-      (gy0,) = torch.autograd.grad(f[0], leaves[0], fbar[0])
-      ybar0.add_(gy0)
-    if ctx.has_g[0] and gbar[0] is not None:
-      ops.terms_adjoint_acc(gbar[0], ybar0)
+      pend_y0.append((ybar, 1.0, True))
+      for j in range(i):
+        if a_ex[i - 1][j] and f[j] is not None:
+          pend_f.setdefault(j, []).append((ybar, a_ex[i - 1][j], False))
+        if a_im[i - 1][j] and ctx.has_g[j]:
+          pend_g.setdefault(j, []).append((ybar, a_im[i - 1][j], False))
+    fbar0 = _fused_gather(ops, pend_f.pop(0, None), dt, obar)
+    gbar0 = _fused_gather(ops, pend_g.pop(0, None), dt, obar, ranged=True)
+    if f[0] is not None and fbar0 is not None:
+      (gy0,) = torch.autograd.grad(f[0], leaves[0], fbar0)
+      pend_y0.append((gy0.contiguous(), 1.0, True))
+    del fbar0
+    ybar0 = _fused_gather(ops, pend_y0, dt, obar)
+    if ctx.has_g[0] and gbar0 is not None:
+      ops.terms_adjoint_acc(gbar0, ybar0)
     ctx.f = ctx.leaves = None
     return None, None, None, ybar0
 
 
 def _fused_imex_step(ops, tab, dt, y0):
   """One IMEX step on a flat state through the fused kernels, or None when
-  the tableau does not fit them (at most 4 terms per group)."""
+  the tableau does not fit them (at most 4 terms per group, 7 stages)."""
   n = len(tab.b_ex)
   rows = [tab.a_ex[i] for i in range(n - 1)] + [tab.b_ex]
   rows_im = [tab.a_im[i] for i in range(n - 1)] + [tab.b_im]
-  if any(sum(1 for c in r if c) > 4 for r in rows + rows_im):
+  if n > 7 or any(sum(1 for c in r if c) > 4 for r in rows + rows_im):
     return None
   buf = y0.flat_buffer()
   if torch.is_grad_enabled() and buf.requires_grad:

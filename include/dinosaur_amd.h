@@ -262,8 +262,9 @@ int dino_imex_stage_adjoint(const void* out_cot, int n_ex, const double* a,
  * degrees > 0 declares the buffers whole Real-layout modal fields of
  * field_size = (2M - 1) * degrees elements: entries with l < m (the part of
  * the dense array outside the triangular truncation, zero in every operand
- * of a masked state) are written as zero without reading any operand.
- * degrees = 0: plain elementwise. */
+ * of a masked state) may be written as zero without reading any operand --
+ * every 16-byte vector of them that lies wholly below the triangle is; the
+ * rest are combined as usual.  degrees = 0: plain elementwise. */
 int dino_imex_stage_ranged(const void* y0, int n_ex, const void* const* f,
                            const double* a, int n_im, const void* const* g,
                            const double* b, double dt, void* out, int64_t n,
@@ -283,8 +284,8 @@ int dino_imex_stage_ranged(const void* y0, int n_ex, const void* const* f,
  * the accumulation of each term's cotangent over the stages that use it
  * fused into the pass.  At most 9 targets; out buffers must not overlap
  * gbar or each other.  field_size / degrees as for dino_imex_stage_ranged:
- * below the triangle gbar is not read, written targets get zero and
- * accumulating ones are left as they are. */
+ * where the kernel skips the triangle, gbar is not read, written targets
+ * get zero and accumulating ones are left as they are. */
 enum {
   DINO_TARGET_ACCUMULATE = 1,
   DINO_TARGET_IDENTITY = 2,
@@ -296,6 +297,26 @@ int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
                                     int64_t im_begin, int64_t im_end,
                                     int64_t g_origin, int64_t field_size,
                                     int degrees, int dtype, void* stream);
+
+/* One cotangent of a reverse IMEX sweep gathered from all its sources in one
+ * pass: out = v_0 + v_1 + ... + v_{n_src-1} in source order, with
+ *   v_k = src[k]                   (identity[k] != 0: a y0 contribution),
+ *   v_k = coef[k] * (dt * src[k])  (otherwise: a stage coefficient),
+ * the same roundings and order as one dino_imex_stage_adjoint_targets pass
+ * writing v_0 followed by passes accumulating v_1, v_2, ... -- so a sweep
+ * may wait until a term's cotangent is needed and form it once instead of
+ * reading and writing it at every stage (time_integration.py:351-370 under
+ * reverse-mode differentiation).  Sources are full-size (n elements);
+ * ranged != 0 writes a ranged implicit cotangent (indexing as in
+ * dino_imex_stage_ranged, nothing written outside [im_begin, im_end)).
+ * field_size / degrees as for dino_imex_stage_ranged: zero where the kernel
+ * skips the triangle.  1 to 8 sources; out must not overlap them. */
+int dino_imex_adjoint_gather(const void* const* src, const double* coef,
+                             const int* identity, int n_src, double dt,
+                             void* out, int64_t n, int ranged,
+                             int64_t im_begin, int64_t im_end,
+                             int64_t g_origin, int64_t field_size,
+                             int degrees, int dtype, void* stream);
 
 /* Fused nodal nonlinear terms of PrimitiveEquationsSigma.explicit_terms.
  * Replaces compute_diagnostic_state_sigma and the nodal parts of

@@ -329,6 +329,63 @@ def test_fused_imex_stage_matches_elementwise_passes(gpu_device):
     assert torch.allclose(x, y, rtol=1e-6, atol=0)
 
 
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
+def test_adjoint_gather_matches_target_passes(dtype, gpu_device):
+  """dino_imex_adjoint_gather forms a cotangent from several sources with the
+  roundings and order of one dino_imex_stage_adjoint_targets pass writing the
+  first contribution followed by passes accumulating the others: bitwise
+  equal for full and ranged (implicit-range) outputs, identity and scaled
+  sources, vector and scalar paths (including the skipped parts below the
+  triangle), and nothing written outside the range (T21 Real-layout fields,
+  4 levels + tracer)."""
+  import ctypes
+  from dinosaur_amd import _lib
+  print(f'[test] adjoint gather {dtype}', flush=True)
+  lib = _lib.lib()
+  R, L, F = 43, 23, 17
+  fs = R * L
+  n = F * fs
+  ib, ie = 4 * fs, 13 * fs
+  go = ib - ib % 4
+  code = _lib.DINO_F64 if dtype == torch.float64 else _lib.DINO_F32
+  gen = torch.Generator(device=gpu_device).manual_seed(3)
+  coef = [0.5, -0.25, 1.0 / 3.0, 2.0]
+  dt = 0.0123
+  stream = ctypes.c_void_p(torch.cuda.current_stream(gpu_device).cuda_stream)
+  P = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])  # noqa: E731
+  D = lambda cs: (ctypes.c_double * len(cs))(*cs)  # noqa: E731
+  I = lambda xs: (ctypes.c_int * len(xs))(*xs)  # noqa: E731
+  for shift in (0, 1):  # 1: buffers off 16-byte alignment, the scalar path
+    srcs = [torch.randn(n + shift, generator=gen, device=gpu_device, dtype=dtype)[shift:]
+            for _ in range(4)]
+    for ranged in (0, 1):
+      size = ie - go if ranged else n
+      for ident in ([1, 1, 1, 1], [0, 0, 0, 0], [1, 0, 1, 0]):
+        nan = lambda: torch.full((size + shift,), float('nan'), device=gpu_device,  # noqa: E731
+                                 dtype=dtype)[shift:]
+        ref, out = nan(), nan()
+        for k, src in enumerate(srcs):
+          kind = ((_lib.TARGET_IDENTITY if ident[k] else 0) |
+                  (_lib.TARGET_RANGED if ranged else 0) | (_lib.TARGET_ACCUMULATE if k else 0))
+          _lib.check(lib.dino_imex_stage_adjoint_targets(
+              ctypes.c_void_p(src.data_ptr()), 1, P([ref]), D([coef[k]]), I([kind]), dt, n, ib,
+              ie, go, fs, L, code, stream), 'targets')
+        _lib.check(lib.dino_imex_adjoint_gather(
+            P(srcs), D(coef), I(ident), 4, dt, ctypes.c_void_p(out.data_ptr()), n, ranged, ib,
+            ie, go, fs, L, code, stream), 'gather')
+        if ranged:
+          pad = ib - go
+          assert torch.isnan(out[:pad]).all() and torch.isnan(ref[:pad]).all()
+          assert torch.equal(out[pad:], ref[pad:]), (shift, ident)
+        else:
+          assert torch.equal(out, ref), (shift, ident)
+  for bad in (0, 9):
+    rc = lib.dino_imex_adjoint_gather(P(srcs), D(coef * 3), I([0] * 12), bad, dt,
+                                      ctypes.c_void_p(srcs[0].data_ptr()), n, 0, ib, ie, go, fs,
+                                      L, code, stream)
+    assert rc == _lib.DINO_EINVAL
+
+
 @pytest.mark.parametrize('keep', [0, 1, 'auto'])
 def test_checkpointed_rollout_gradient_matches_plain_autograd(keep, gpu_device):
   """checkpointed_rollout (segments recomputed in the backward pass, or --

@@ -21,7 +21,7 @@
 // with UP = (l+1) a_lm, DN = -l b_lm (cos_lat_d_dlat, :1063-1083) and
 // UP2 = (l-1) a_lm, DN2 = -(l+2) b_lm (sec_lat_d_dlat_cos2, :1085-1092).
 // dlon is antisymmetric; the adjoints of cldl/sdlc are the transposed
-// two-point stencils.  One thread per (member, r, l) walks all levels.
+// two-point stencils.  One thread per (member, r, l) walks a chunk of levels.
 #include <cstring>
 #include <vector>
 
@@ -34,6 +34,10 @@ struct dino_modal {
   // jr[R] (coefficient), pr[R] (partner row, stored as a number)
   double* c64;
   float* c32;
+  // dense positions r * L + l with l >= leg_tile_start(m, L), m = (r + 1) / 2:
+  // the part of a field the inverse Legendre kernels read, in row order
+  int* tile_pos;
+  int64_t n_tile_pos;
 };
 
 namespace dino {
@@ -110,9 +114,18 @@ inline EpiGroups epi_groups(int K, int nt, bool moist, int64_t E) {
 
 struct MP {
   int R, L, K, nt, hum;
+  int kc, nkc;  // levels per thread and level chunks: blockIdx.y = member * nkc + chunk
   int64_t RL;
   double a, g;
 };
+
+// Levels one thread walks: chunks of 4 levels give the launch ~9x more
+// threads than whole columns at 37 levels, and 10-18% shorter kernels at
+// TL255 x 16 members (more loads in flight; the per-position stencil
+// coefficients are re-read per chunk, from cache).
+#ifndef DINO_MODAL_LEVELS_PER_THREAD
+#define DINO_MODAL_LEVELS_PER_THREAD 4
+#endif
 
 // Stencil of one thread's (r, l): the bidiagonal-in-l operators read the
 // neighbours l - 1 and l + 1.  Out-of-range neighbours are loaded from a
@@ -178,17 +191,21 @@ struct Stencil {
 };
 
 // ---------------------------------------------------------------- prep
+// Only the inverse Legendre transform reads the output, from the first degree
+// of its tiles on: threads walk the listed positions l >= leg_tile_start(m, L)
+// only (full waves; the degrees below are not written).
 template <typename T>
 __global__ __launch_bounds__(256) void prep_kernel(const T* __restrict__ st, T* __restrict__ out,
-                                                   const T* __restrict__ cst, StateGroups sg,
-                                                   PrepGroups pg, MP p) {
-  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t e = blockIdx.y;
-  if (pos >= p.RL) return;
+                                                   const T* __restrict__ cst,
+                                                   const int* __restrict__ tile_pos,
+                                                   int64_t n_pos, StateGroups sg, PrepGroups pg,
+                                                   MP p) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = blockIdx.y / p.nkc;
+  const int kch = blockIdx.y % p.nkc;
+  if (q >= n_pos) return;
+  const int64_t pos = tile_pos[q];
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);
-  // only the inverse Legendre transform reads this output, from the first
-  // degree of its tiles on: the degrees below are not written
-  if (l < leg_tile_start((r + 1) >> 1, p.L)) return;
   const MOff o = moff(p.R, p.L);
   Stencil<T> S(cst, o, p.R, p.L, r, l);
   const int K = p.K;
@@ -203,7 +220,8 @@ __global__ __launch_bounds__(256) void prep_kernel(const T* __restrict__ st, T* 
     return tri ? zb : base + (off + e * lev + k) * p.RL;
   };
   auto W = [&](T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
-  for (int k = 0; k < K; ++k) {
+  const int k1 = (kch + 1) * p.kc < K ? (kch + 1) * p.kc : K;
+  for (int k = kch * p.kc; k < k1; ++k) {
     const T* z = F(st, sg.z, K, k);
     const T* d = F(st, sg.d, K, k);
     T zv = z[pos], dv = d[pos];
@@ -225,6 +243,7 @@ __global__ __launch_bounds__(256) void prep_kernel(const T* __restrict__ st, T* 
       *(W(out, pg.qy, K, k) + pos) = S.cldl(q) * inva;
     }
   }
+  if (kch != 0) return;  // the single-level fields, once per member
   const T* lp = F(st, sg.lnps, 1, 0);
   *(W(out, pg.gx, 1, 0) + pos) = S.dlon(lp) * inva;
   *(W(out, pg.gy, 1, 0) + pos) = S.cldl(lp) * inva;
@@ -237,7 +256,8 @@ __global__ __launch_bounds__(256) void prep_adjoint_kernel(const T* __restrict__
                                                            const T* __restrict__ cst,
                                                            StateGroups sg, PrepGroups pg, MP p) {
   const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t e = blockIdx.y;
+  const int64_t e = blockIdx.y / p.nkc;
+  const int kch = blockIdx.y % p.nkc;
   if (pos >= p.RL) return;
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);
   const MOff o = moff(p.R, p.L);
@@ -254,7 +274,8 @@ __global__ __launch_bounds__(256) void prep_adjoint_kernel(const T* __restrict__
     return tri ? zb : base + (off + e * lev + k) * p.RL;
   };
   auto W = [&](T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
-  for (int k = 0; k < K; ++k) {
+  const int k1 = (kch + 1) * p.kc < K ? (kch + 1) * p.kc : K;
+  for (int k = kch * p.kc; k < k1; ++k) {
     const T* ub = F(gout, pg.u, K, k);
     const T* vb = F(gout, pg.v, K, k);
     // U = (dlon(chi) - cldl(psi)) / a, V = (cldl(chi) + dlon(psi)) / a
@@ -271,6 +292,7 @@ __global__ __launch_bounds__(256) void prep_adjoint_kernel(const T* __restrict__
       *(W(gst, sg.tr[i], K, k) + pos) = v;
     }
   }
+  if (kch != 0) return;  // the single-level field, once per member
   T lb = (-S.dlon(F(gout, pg.gx, 1, 0)) + S.cldl_t(F(gout, pg.gy, 1, 0))) * inva;
   if (p.hum >= 0) lb += cst[o.ev + l] * *(F(gout, pg.lp, 1, 0) + pos);
   *(W(gst, sg.lnps, 1, 0) + pos) = lb;
@@ -284,7 +306,8 @@ __global__ __launch_bounds__(256) void epilogue_kernel(const T* __restrict__ in,
                                                        const T* __restrict__ cst, EpiGroups eg,
                                                        StateGroups sg, MP p) {
   const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t e = blockIdx.y;
+  const int64_t e = blockIdx.y / p.nkc;
+  const int kch = blockIdx.y % p.nkc;
   if (pos >= p.RL) return;
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);
   const MOff o = moff(p.R, p.L);
@@ -303,7 +326,8 @@ __global__ __launch_bounds__(256) void epilogue_kernel(const T* __restrict__ in,
   };
   auto W = [&](T* base, int64_t off, int lev, int k) { return base + (off + e * lev + k) * p.RL; };
   const bool moist = p.hum >= 0;
-  for (int k = 0; k < K; ++k) {
+  const int k1 = (kch + 1) * p.kc < K ? (kch + 1) * p.kc : K;
+  for (int k = kch * p.kc; k < k1; ++k) {
     const T* cu = F(in, eg.cu, K, k);
     const T* cv = F(in, eg.cv, K, k);
     // -curl_cos(cu, cv) = (sdlc(cu) - dlon(cv)) / a ; -div_cos = -(dlon(cu) + sdlc(cv)) / a
@@ -325,22 +349,24 @@ __global__ __launch_bounds__(256) void epilogue_kernel(const T* __restrict__ in,
       *(W(st, sg.tr[i], K, k) + pos) = v * mask;
     }
   }
-  *(W(st, sg.lnps, 1, 0) + pos) = *(F(in, eg.lsp, 1, 0) + pos) * mask;
+  if (kch == 0) *(W(st, sg.lnps, 1, 0) + pos) = *(F(in, eg.lsp, 1, 0) + pos) * mask;
 }
 
+// Output positions as for prep_kernel: the inverse Legendre transform of the
+// adjoint reads it from the first degree of its tiles on.
 template <typename T>
 __global__ __launch_bounds__(256) void epilogue_adjoint_kernel(const T* __restrict__ gst,
                                                                T* __restrict__ gin,
                                                                const T* __restrict__ cst,
-                                                               EpiGroups eg, StateGroups sg,
-                                                               MP p) {
-  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t e = blockIdx.y;
-  if (pos >= p.RL) return;
+                                                               const int* __restrict__ tile_pos,
+                                                               int64_t n_pos, EpiGroups eg,
+                                                               StateGroups sg, MP p) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = blockIdx.y / p.nkc;
+  const int kch = blockIdx.y % p.nkc;
+  if (q >= n_pos) return;
+  const int64_t pos = tile_pos[q];
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);
-  // only the inverse Legendre transform reads this output, from the first
-  // degree of its tiles on: the degrees below are not written
-  if (l < leg_tile_start((r + 1) >> 1, p.L)) return;
   const MOff o = moff(p.R, p.L);
   Stencil<T> S(cst, o, p.R, p.L, r, l);
   const int K = p.K;
@@ -366,7 +392,8 @@ __global__ __launch_bounds__(256) void epilogue_adjoint_kernel(const T* __restri
     return (S.hm ? a : (T)0) + (S.hp ? b : (T)0);
   };
   const bool moist = p.hum >= 0;
-  for (int k = 0; k < K; ++k) {
+  const int k1 = (kch + 1) * p.kc < K ? (kch + 1) * p.kc : K;
+  for (int k = kch * p.kc; k < k1; ++k) {
     const T* zb = F(gst, sg.z, K, k);
     const T* db = F(gst, sg.d, K, k);
     const T* tb = F(gst, sg.t, K, k);
@@ -393,7 +420,7 @@ __global__ __launch_bounds__(256) void epilogue_adjoint_kernel(const T* __restri
       *(W(gin, eg.dc, K, k) + pos) = -dm;
     }
   }
-  *(W(gin, eg.lsp, 1, 0) + pos) = MB(F(gst, sg.lnps, 1, 0));
+  if (kch == 0) *(W(gin, eg.lsp, 1, 0) + pos) = MB(F(gst, sg.lnps, 1, 0));
 }
 
 template <typename T>
@@ -409,6 +436,8 @@ MP make_mp(const dino_modal* m) {
   p.K = m->K;
   p.nt = m->nt;
   p.hum = m->hum;
+  p.kc = DINO_MODAL_LEVELS_PER_THREAD < m->K ? DINO_MODAL_LEVELS_PER_THREAD : m->K;
+  p.nkc = (m->K + p.kc - 1) / p.kc;
   p.RL = (int64_t)m->R * m->L;
   p.a = m->radius;
   p.g = m->gravity;
@@ -419,14 +448,21 @@ template <typename T>
 int run(int which, const dino_modal* m, const void* a, const void* b, void* out, int64_t E,
         hipStream_t s) {
   MP p = make_mp(m);
+  // level chunks: as many as the grid's y extent allows for E members
+  while ((int64_t)p.nkc * E > 65535) {
+    p.kc *= 2;
+    p.nkc = (m->K + p.kc - 1) / p.kc;
+  }
   bool moist = m->hum >= 0;
   StateGroups sg = state_groups(m->K, m->nt, E);
-  dim3 grid((unsigned)((p.RL + 255) / 256), (unsigned)E);
+  const unsigned gy = (unsigned)(E * p.nkc);
+  dim3 grid((unsigned)((p.RL + 255) / 256), gy);
+  dim3 tgrid((unsigned)((m->n_tile_pos + 255) / 256), gy);
   const T* c = consts<T>(m);
   switch (which) {
     case 0:
-      prep_kernel<T><<<grid, 256, 0, s>>>((const T*)a, (T*)out, c, sg,
-                                          prep_groups(m->K, m->nt, moist, E), p);
+      prep_kernel<T><<<tgrid, 256, 0, s>>>((const T*)a, (T*)out, c, m->tile_pos, m->n_tile_pos,
+                                           sg, prep_groups(m->K, m->nt, moist, E), p);
       break;
     case 1:
       prep_adjoint_kernel<T><<<grid, 256, 0, s>>>((const T*)a, (T*)out, c, sg,
@@ -437,8 +473,9 @@ int run(int which, const dino_modal* m, const void* a, const void* b, void* out,
                                               epi_groups(m->K, m->nt, moist, E), sg, p);
       break;
     default:
-      epilogue_adjoint_kernel<T><<<grid, 256, 0, s>>>((const T*)a, (T*)out, c,
-                                                      epi_groups(m->K, m->nt, moist, E), sg, p);
+      epilogue_adjoint_kernel<T><<<tgrid, 256, 0, s>>>((const T*)a, (T*)out, c, m->tile_pos,
+                                                       m->n_tile_pos,
+                                                       epi_groups(m->K, m->nt, moist, E), sg, p);
   }
   DINO_LAUNCH_CHECK("modal kernel");
   return DINO_OK;
@@ -456,8 +493,8 @@ extern "C" int dino_modal_create(int R, int L, int K, int n_tracers, int humidit
                                  dino_modal** out) {
   DINO_REQUIRE(out, DINO_EINVAL, "dino_modal_create: out is NULL");
   *out = nullptr;
-  DINO_REQUIRE(R >= 1 && R % 2 == 1 && L >= 2 && K >= 1, DINO_EINVAL,
-               "invalid modal shape (%d, %d) x %d levels", R, L, K);
+  DINO_REQUIRE(R >= 1 && R % 2 == 1 && L >= 2 && K >= 1 && (int64_t)R * L < (int64_t)1 << 31,
+               DINO_EINVAL, "invalid modal shape (%d, %d) x %d levels", R, L, K);
   DINO_REQUIRE(n_tracers >= 0 && n_tracers <= MAXTR, DINO_EINVAL,
                "at most %d tracers are supported", MAXTR);
   DINO_REQUIRE(humidity_index < n_tracers, DINO_EINVAL, "humidity index out of range");
@@ -490,6 +527,9 @@ extern "C" int dino_modal_create(int R, int L, int K, int n_tracers, int humidit
     }
   }
   std::vector<float> hf(h.begin(), h.end());
+  std::vector<int> tp;
+  for (int r = 0; r < R; ++r)
+    for (int l = leg_tile_start((r + 1) / 2, L); l < L; ++l) tp.push_back(r * L + l);
   dino_modal* m = new dino_modal();
   std::memset(m, 0, sizeof(*m));
   m->R = R;
@@ -503,9 +543,14 @@ extern "C" int dino_modal_create(int R, int L, int K, int n_tracers, int humidit
   if (e == hipSuccess) e = hipMalloc(&m->c32, sizeof(float) * o.total);
   if (e == hipSuccess) e = hipMemcpy(m->c64, h.data(), sizeof(double) * o.total, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(m->c32, hf.data(), sizeof(float) * o.total, hipMemcpyHostToDevice);
+  m->n_tile_pos = (int64_t)tp.size();
+  if (e == hipSuccess) e = hipMalloc(&m->tile_pos, sizeof(int) * tp.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(m->tile_pos, tp.data(), sizeof(int) * tp.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     if (m->c64) (void)hipFree(m->c64);
     if (m->c32) (void)hipFree(m->c32);
+    if (m->tile_pos) (void)hipFree(m->tile_pos);
     delete m;
     return hip_fail(e, "dino_modal_create");
   }
@@ -517,6 +562,7 @@ extern "C" int dino_modal_destroy(dino_modal* m) {
   if (m) {
     if (m->c64) (void)hipFree(m->c64);
     if (m->c32) (void)hipFree(m->c32);
+    if (m->tile_pos) (void)hipFree(m->tile_pos);
     delete m;
   }
   return DINO_OK;

@@ -71,7 +71,8 @@ def _parse():
 STAGES = ('legendre_forward', 'legendre_inverse', 'dft_forward', 'dft_inverse')
 IMPLICIT_STAGES = ('implicit_solve', 'implicit_terms')
 # fields per member in one explicit-terms evaluation of the moist L37 state
-FWD_FIELDS, INV_FIELDS = 667, 374
+# (17 per level + ln ps forward, 10 per level + 3 inverse)
+FWD_FIELDS, INV_FIELDS = 630, 374
 
 
 def measure_implicit(torch, eq, state, stage, reps=10):

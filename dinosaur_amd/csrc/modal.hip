@@ -9,7 +9,8 @@
 //           equations laplacian(ln ps) and cos(lat) grad q
 //           (primitive_equations.py:168-225, 861-963).
 // epilogue: the batched forward-transform output -> tendencies:
-//           curl / divergence of the combined fluxes, kinetic energy and
+//           curl / divergence of the combined fluxes, kinetic energy (with the
+//           moist geopotential correction folded in by the nodal kernel) and
 //           orography Laplacians, div_sec_lat of the advective fluxes, the
 //           humidity corrections and the top-wavenumber clip
 //           (primitive_equations.py:1001-1059, 778-805, 1126-1194).
@@ -72,7 +73,7 @@ struct StateGroups {  // state leaves in one buffer: vort, div, temp, lnps, trac
   int64_t z, d, t, lnps, tr[MAXTR], n;
 };
 struct EpiGroups {  // forward-transform output (fields)
-  int64_t cu, cv, ke, te, ut, vt, lsp, trn[MAXTR], tu[MAXTR], tv[MAXTR], curl, geo, dc, n;
+  int64_t cu, cv, ke, te, ut, vt, lsp, trn[MAXTR], tu[MAXTR], tv[MAXTR], curl, dc, n;
 };
 
 inline PrepGroups prep_groups(int K, int nt, bool moist, int64_t E) {
@@ -107,7 +108,7 @@ inline EpiGroups epi_groups(int K, int nt, bool moist, int64_t E) {
   g.cu = take(K); g.cv = take(K); g.ke = take(K); g.te = take(K); g.ut = take(K);
   g.vt = take(K); g.lsp = take(1);
   for (int i = 0; i < nt; ++i) { g.trn[i] = take(K); g.tu[i] = take(K); g.tv[i] = take(K); }
-  if (moist) { g.curl = take(K); g.geo = take(K); g.dc = take(K); }
+  if (moist) { g.curl = take(K); g.dc = take(K); }
   g.n = o;
   return g;
 }
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(256) void epilogue_kernel(const T* __restrict__ in,
              (S.dlon(F(in, eg.ut, K, k)) + S.sdlc(F(in, eg.vt, K, k))) * inva;
     if (moist) {
       vort += *(F(in, eg.curl, K, k) + pos);
-      div += -ev * *(F(in, eg.geo, K, k) + pos) - *(F(in, eg.dc, K, k) + pos);
+      div -= *(F(in, eg.dc, K, k) + pos);
     }
     *(W(st, sg.z, K, k) + pos) = vort * mask;
     *(W(st, sg.d, K, k) + pos) = div * mask;
@@ -416,7 +417,6 @@ __global__ __launch_bounds__(256) void epilogue_adjoint_kernel(const T* __restri
     }
     if (moist) {
       *(W(gin, eg.curl, K, k) + pos) = zm;
-      *(W(gin, eg.geo, K, k) + pos) = -ev * dm;
       *(W(gin, eg.dc, K, k) + pos) = -dm;
     }
   }

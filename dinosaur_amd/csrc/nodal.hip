@@ -23,7 +23,11 @@
 // and level k is field off[g] + e * levels(g) + k.
 //
 // Input groups : Z D T TR[0..nt) U V GX GY [LP QX QY]
-// Output groups: CU CV KE TEMP UT VT LSP (TRN TU TV)[0..nt) [CURL GEO DC]
+// Output groups: CU CV KE TEMP UT VT LSP (TRN TU TV)[0..nt) [CURL DC]
+// For moist equations KE holds the kinetic energy plus the virtual-temperature
+// geopotential: both enter the divergence tendency as -laplacian(.)
+// (primitive_equations.py:1001-1059, 1126-1194), so one field carries their
+// sum through the forward transform instead of two.
 #include <cstring>
 
 #include "dino_common.h"
@@ -72,7 +76,7 @@ struct Groups {
   // input group offsets (in fields of I*J) for member stride computation
   int64_t z, d, t, tr[MAXTR], u, v, gx, gy, lp, qx, qy;
   // output group offsets
-  int64_t cu, cv, ke, te, ut, vt, lsp, trn[MAXTR], tu[MAXTR], tv[MAXTR], curl, geo, dc;
+  int64_t cu, cv, ke, te, ut, vt, lsp, trn[MAXTR], tu[MAXTR], tv[MAXTR], curl, dc;
   int64_t n_in, n_out;
 };
 
@@ -114,7 +118,6 @@ inline Groups make_groups(int K, int nt, int moist, int64_t E) {
   }
   if (moist) {
     g.curl = take(K);
-    g.geo = take(K);
     g.dc = take(K);
   }
   g.n_out = o;
@@ -243,7 +246,8 @@ __global__ __launch_bounds__(256) void nodal_forward_kernel(
     const T zt = z + fc;
     *O_(g.cu, K, k) = -v0 * zt * s + ((iva ? -vadv_u : (T)0) + rt * gx) * s;
     *O_(g.cv, K, k) = u0 * zt * s + ((iva ? -vadv_v : (T)0) + rt * gy) * s;
-    *O_(g.ke, K, k) = (u0 * u0 + v0 * v0) * s / (T)2;
+    const T kinetic = (u0 * u0 + v0 * v0) * s / (T)2;
+    if (!moist) *O_(g.ke, K, k) = kinetic;
     *O_(g.te, K, k) = temp;
     *O_(g.ut, K, k) = u0 * t0 * s;
     *O_(g.vt, K, k) = v0 * t0 * s;
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(256) void nodal_forward_kernel(
       *O_(g.dc, K, k) = cf * s * (qx * gx + qy * gy) + q * lp * tref * (T)p.rvr;
       T td = q * (t0 + tref) * (T)p.gcr1;
       if (k >= 1) GP += cst[o.gp + k] * td;
-      *O_(g.geo, K, k) = cst[o.gd + k] * td + (GT - GP);
+      *O_(g.ke, K, k) = kinetic + (cst[o.gd + k] * td + (GT - GP));
     }
     fu_m = fu;
     fv_m = fv;
@@ -373,7 +377,7 @@ __global__ __launch_bounds__(256) void nodal_adjoint_kernel(
     }
     b.curl = moist ? B_(g.curl, K, k) : (T)0;
     b.dc = moist ? B_(g.dc, K, k) : (T)0;
-    b.geo = moist ? B_(g.geo, K, k) : (T)0;
+    b.geo = moist ? b.ke : (T)0;  // KE carries the geopotential too
   };
   // cotangents of vadv(W, x) at a level
   auto vb_u = [&](const BL& b) -> T { return iva ? -s * b.cu : (T)0; };
@@ -437,7 +441,7 @@ __global__ __launch_bounds__(256) void nodal_adjoint_kernel(
         TFb += cst[o.ss + k] * wbar(k, xa, xb, ba, bb);
         TEb += cst[o.ss + k] * xbar(k, ba, bb);
       }
-      if (moist) GEOT += B_(g.geo, K, k);
+      if (moist) GEOT += B_(g.ke, K, k);
       xa = xb;
       ba = bb;
     }

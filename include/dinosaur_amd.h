@@ -330,7 +330,10 @@ int dino_imex_adjoint_gather(const void* const* src, const double* coef,
  * (members, levels): Z D T TRACER[0..n) U V GX GY [LAP_LNPS QX QY]
  * (levels K except GX, GY, LAP_LNPS: 1; the bracketed groups only when
  * humidity_index >= 0).  Output groups: CU CV KE TEMP UT VT LSP
- * (TRN TU TV)[0..n) [CURL GEO DC].  Per-level constants are host FP64
+ * (TRN TU TV)[0..n) [CURL DC]; with humidity KE holds the kinetic energy
+ * plus the virtual-temperature geopotential, which the divergence tendency
+ * takes through the same -laplacian (the reference transforms the two
+ * separately).  Per-level constants are host FP64
  * arrays of length K (c2c_inv and sum_sigma use the first K-1 entries);
  * sec2 and coriolis have length J; geo_off[k] = G[0][k] (k >= 1, geo_off[0]
  * unused) and geo_diag[k] = G[k][k] of the geopotential matrix (may be NULL

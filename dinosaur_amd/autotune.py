@@ -9,8 +9,9 @@ the choice up by batch size (same power-of-two bucket, else the nearest tuned
 bucket) and launch with it, so a tuned plan needs no further calls.
 
 Choices can be kept in a JSON file (`cache` argument or the DINO_AUTOTUNE_CACHE
-environment variable), keyed by device name, grid, spacing, data type,
-precision and batch bucket, so a later process loads them without timing.
+environment variable), keyed by device architecture and CU count, grid,
+spacing, data type, precision and batch bucket, so a later process loads them
+without timing.
 
 Every schedule computes the same contraction; they differ in speed and in the
 FP32 accumulation order only.  FP64 data always runs the FP64 kernels, so
@@ -71,8 +72,11 @@ def lookup(tuned: dict, batch: int, dtype_code: int, precision: int):
 
 
 def _file_key(plan, spacing, b, precision) -> str:
-  name = torch.cuda.get_device_name(plan.device)
-  return f'{name}|{plan.M}x{plan.L}x{plan.I}x{plan.J}|{spacing}|f32|{precision}|b{b}'
+  # the architecture and CU count, not the marketing name, which some tool
+  # environments (profilers) report as empty
+  props = torch.cuda.get_device_properties(plan.device)
+  dev = f'{getattr(props, "gcnArchName", "") or props.name}x{props.multi_processor_count}'
+  return f'{dev}|{plan.M}x{plan.L}x{plan.I}x{plan.J}|{spacing}|f32|{precision}|b{b}'
 
 
 def _load(path):

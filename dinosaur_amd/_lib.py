@@ -24,6 +24,7 @@ SCHED_AUTO, SCHED_ROWS, SCHED_TILED, SCHED_TILED_NARROW = 0, 1, 2, 3
 SCHED_PERSISTENT, SCHED_PER_TILE = 1, 2
 SCHED_MPAIRED = 0x100
 TARGET_ACCUMULATE, TARGET_IDENTITY, TARGET_RANGED = 1, 2, 4
+GATHER_IDENTITY, GATHER_SCALED = 1, 2
 SHT_TRIANGLE_ONLY = 0x100
 PRECISIONS = {'native': PREC_NATIVE, 'float32': PREC_NATIVE, 'float64': PREC_NATIVE,
               'f16x3': PREC_F16X3, 'bf16x3': PREC_BF16X3}
@@ -73,7 +74,10 @@ SIGNATURES = {
     'dino_imex_stage_adjoint_targets': (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p, _c_d, _c_i64, _c_i64,
                                                _c_i64, _c_i64, _c_i64, _c_i, _c_i, _c_p]),
     'dino_imex_adjoint_gather': (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_d, _c_p, _c_i64, _c_i, _c_i64,
-                                        _c_i64, _c_i64, _c_i64, _c_i, _c_i, _c_p]),
+                                        _c_i64, _c_i64, _c_i64, _c_i, _c_p, _c_i64, _c_i, _c_p]),
+    'dino_imex_stage_ranged_scaled': (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_i, _c_p, _c_p, _c_d, _c_p,
+                                             _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_i, _c_p,
+                                             _c_i64, _c_i, _c_p]),
     'dino_implicit_terms_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, _c_i,
                                            _c_p]),
     'dino_nodal_create': (_c_i, [_c_i, _c_i, _c_i, _c_i, _c_i, ctypes.POINTER(_c_i), _c_i, _c_i,

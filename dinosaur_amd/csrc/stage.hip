@@ -55,6 +55,8 @@ struct StageArgs {
   int64_t ib, ie, go;
   int64_t trl;  // > 0: Real-layout modal fields of trl = R * L elements ...
   int tl;       // ... of tl degrees per row; elements with l < m are zero
+  const T* ls;  // non-null: the result is multiplied by ls[i % sl] (a
+  int64_t sl;   // per-degree filter applied to the output, one rounding)
 };
 
 template <typename T>
@@ -107,6 +109,7 @@ __device__ __forceinline__ T combine_scalar(const StageArgs<T>& a, int64_t i) {
   T y = a.y0[i];
   if (a.n[0]) y = y + a.dt * group_sum<T, T>(a, 0, i);
   if (a.n[1] && i >= a.ib && i < a.ie) y = y + a.dt * group_sum<T, T>(a, 1, i - a.go);
+  if (a.ls) y = y * a.ls[i % a.sl];
   return y;
 }
 
@@ -139,6 +142,14 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a, T* __restric
             const int64_t i = i0 + e;
             if (i >= a.ib && i < a.ie) y[e] = y[e] + a.dt * group_sum<T, T>(a, 1, i - a.go);
           }
+        }
+      }
+      if (a.ls) {
+        int64_t l = i0 % a.sl;
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+          y[e] = y[e] * a.ls[l];
+          l = l + 1 == a.sl ? 0 : l + 1;
         }
       }
       reinterpret_cast<V*>(out)[k] = y;
@@ -254,19 +265,35 @@ template <typename T>
 struct GatherArgs {
   const T* src[kMaxSources];
   T c[kMaxSources];
-  int ident[kMaxSources];
+  int flags[kMaxSources];  // DINO_GATHER_IDENTITY | DINO_GATHER_SCALED
   int n;
   T dt;
   int64_t off, ib, ie;
   int64_t trl;
   int tl;
+  const T* ls;  // per-degree scale of the DINO_GATHER_SCALED sources, ls[i % sl]
+  int64_t sl;
 };
 
+// U = T: idx is the element; U = vector: idx is the vector (elements W idx ..)
 template <typename T, typename U>
 __device__ __forceinline__ U gather_sum(const GatherArgs<T>& a, int64_t idx) {
+  constexpr int W = sizeof(U) / sizeof(T);
   auto term = [&](int k) {
-    const U g = reinterpret_cast<const U*>(a.src[k])[idx];
-    return a.ident[k] ? g : a.c[k] * (a.dt * g);
+    U g = reinterpret_cast<const U*>(a.src[k])[idx];
+    if (a.flags[k] & DINO_GATHER_SCALED) {
+      if constexpr (W == 1) {
+        g = g * a.ls[idx % a.sl];
+      } else {
+        int64_t l = (idx * W) % a.sl;
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+          g[e] = g[e] * a.ls[l];
+          l = l + 1 == a.sl ? 0 : l + 1;
+        }
+      }
+    }
+    return (a.flags[k] & DINO_GATHER_IDENTITY) ? g : a.c[k] * (a.dt * g);
   };
   U s = term(0);
 #pragma unroll
@@ -311,10 +338,12 @@ template <typename T>
 int launch_stage(const void* y0, int n_ex, const void* const* f, const double* a, int n_im,
                  const void* const* g, const double* b, double dt, void* out, int64_t n,
                  int64_t ib, int64_t ie, int64_t go, hipStream_t s, int64_t trl = 0,
-                 int tl = 0) {
+                 int tl = 0, const void* ls = nullptr, int64_t sl = 0) {
   StageArgs<T> args{};
   args.trl = trl;
   args.tl = tl;
+  args.ls = static_cast<const T*>(ls);
+  args.sl = sl;
   args.y0 = static_cast<const T*>(y0);
   args.n[0] = n_ex;
   args.n[1] = n_im;
@@ -407,10 +436,12 @@ int launch_targets(const void* gbar, int nt, void* const* outp, const double* co
 }
 
 template <typename T>
-int launch_gather(const void* const* src, const double* coef, const int* ident, int ns,
+int launch_gather(const void* const* src, const double* coef, const int* flags, int ns,
                   double dt, void* out, int64_t n_out, int64_t off, int64_t ib, int64_t ie,
-                  hipStream_t s, int64_t trl, int tl) {
+                  hipStream_t s, int64_t trl, int tl, const void* ls, int64_t sl) {
   GatherArgs<T> args{};
+  args.ls = static_cast<const T*>(ls);
+  args.sl = sl;
   args.n = ns;
   args.dt = (T)dt;
   args.off = off;
@@ -422,7 +453,7 @@ int launch_gather(const void* const* src, const double* coef, const int* ident, 
   for (int k = 0; k < ns; ++k) {
     args.src[k] = static_cast<const T*>(src[k]);
     args.c[k] = (T)coef[k];
-    args.ident[k] = ident[k] ? 1 : 0;
+    args.flags[k] = flags[k];
     vec = vec && aligned16(src[k]);
   }
   const int W = vec ? Vec<T>::W : 1;
@@ -477,17 +508,21 @@ extern "C" int dino_imex_stage(const void* y0, int n_ex, const void* const* f, c
                              as_stream(stream));
 }
 
-extern "C" int dino_imex_stage_ranged(const void* y0, int n_ex, const void* const* f,
-                                      const double* a, int n_im, const void* const* g,
-                                      const double* b, double dt, void* out, int64_t n,
-                                      int64_t im_begin, int64_t im_end, int64_t g_origin,
-                                      int64_t field_size, int degrees, int dtype,
-                                      void* stream) {
-  const char* name = "dino_imex_stage_ranged";
+namespace dino {
+namespace {
+
+int stage_ranged(const char* name, const void* y0, int n_ex, const void* const* f,
+                 const double* a, int n_im, const void* const* g, const double* b, double dt,
+                 void* out, int64_t n, int64_t im_begin, int64_t im_end, int64_t g_origin,
+                 int64_t field_size, int degrees, const void* scale, int64_t scale_len,
+                 int dtype, void* stream) {
   DINO_REQUIRE(degrees == 0 || (degrees > 0 && field_size > 0 && field_size % degrees == 0 &&
                                 (field_size / degrees) % 2 == 1 && n % field_size == 0),
                DINO_EINVAL, "%s: invalid Real-layout field shape (%lld, %d)", name,
                (long long)field_size, degrees);
+  DINO_REQUIRE(!scale || (scale_len > 0 && n % scale_len == 0), DINO_EINVAL,
+               "%s: the scale length %lld must divide the size %lld", name,
+               (long long)scale_len, (long long)n);
   int rc = check_args(name, n_ex, n_im, a, b, n, dtype);
   if (rc != DINO_OK) return rc;
   rc = check_range(name, n, im_begin, im_end, g_origin);
@@ -497,11 +532,39 @@ extern "C" int dino_imex_stage_ranged(const void* y0, int n_ex, const void* cons
   for (int j = 0; j < n_ex; ++j) DINO_REQUIRE(f[j], DINO_EINVAL, "%s: NULL f", name);
   for (int j = 0; j < n_im; ++j) DINO_REQUIRE(g[j], DINO_EINVAL, "%s: NULL g", name);
   if (n == 0) return DINO_OK;
+  const int64_t trl = degrees ? field_size : 0;
   if (dtype == DINO_F64)
     return launch_stage<double>(y0, n_ex, f, a, n_im, g, b, dt, out, n, im_begin, im_end,
-                                g_origin, as_stream(stream), degrees ? field_size : 0, degrees);
+                                g_origin, as_stream(stream), trl, degrees, scale, scale_len);
   return launch_stage<float>(y0, n_ex, f, a, n_im, g, b, dt, out, n, im_begin, im_end,
-                             g_origin, as_stream(stream), degrees ? field_size : 0, degrees);
+                             g_origin, as_stream(stream), trl, degrees, scale, scale_len);
+}
+
+}  // namespace
+}  // namespace dino
+
+extern "C" int dino_imex_stage_ranged(const void* y0, int n_ex, const void* const* f,
+                                      const double* a, int n_im, const void* const* g,
+                                      const double* b, double dt, void* out, int64_t n,
+                                      int64_t im_begin, int64_t im_end, int64_t g_origin,
+                                      int64_t field_size, int degrees, int dtype,
+                                      void* stream) {
+  return stage_ranged("dino_imex_stage_ranged", y0, n_ex, f, a, n_im, g, b, dt, out, n,
+                      im_begin, im_end, g_origin, field_size, degrees, nullptr, 0, dtype,
+                      stream);
+}
+
+extern "C" int dino_imex_stage_ranged_scaled(const void* y0, int n_ex, const void* const* f,
+                                             const double* a, int n_im, const void* const* g,
+                                             const double* b, double dt, void* out, int64_t n,
+                                             int64_t im_begin, int64_t im_end,
+                                             int64_t g_origin, int64_t field_size, int degrees,
+                                             const void* degree_scale, int64_t scale_len,
+                                             int dtype, void* stream) {
+  DINO_REQUIRE(degree_scale, DINO_EINVAL, "dino_imex_stage_ranged_scaled: NULL scale");
+  return stage_ranged("dino_imex_stage_ranged_scaled", y0, n_ex, f, a, n_im, g, b, dt, out, n,
+                      im_begin, im_end, g_origin, field_size, degrees, degree_scale, scale_len,
+                      dtype, stream);
 }
 
 extern "C" int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
@@ -540,10 +603,11 @@ extern "C" int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
 }
 
 extern "C" int dino_imex_adjoint_gather(const void* const* src, const double* coef,
-                                        const int* identity, int n_src, double dt, void* out,
+                                        const int* flags, int n_src, double dt, void* out,
                                         int64_t n, int ranged, int64_t im_begin,
                                         int64_t im_end, int64_t g_origin, int64_t field_size,
-                                        int degrees, int dtype, void* stream) {
+                                        int degrees, const void* degree_scale,
+                                        int64_t scale_len, int dtype, void* stream) {
   const char* name = "dino_imex_adjoint_gather";
   DINO_REQUIRE(degrees == 0 || (degrees > 0 && field_size > 0 && field_size % degrees == 0 &&
                                 (field_size / degrees) % 2 == 1 && n % field_size == 0),
@@ -551,11 +615,19 @@ extern "C" int dino_imex_adjoint_gather(const void* const* src, const double* co
                (long long)field_size, degrees);
   DINO_REQUIRE(n_src >= 1 && n_src <= kMaxSources, DINO_EINVAL,
                "%s: need 1 to %d sources (got %d)", name, kMaxSources, n_src);
-  DINO_REQUIRE(src && coef && identity && out, DINO_EINVAL, "%s: NULL argument", name);
+  DINO_REQUIRE(src && coef && flags && out, DINO_EINVAL, "%s: NULL argument", name);
   DINO_REQUIRE(n >= 0, DINO_EINVAL, "%s: negative size", name);
   DINO_REQUIRE(dtype == DINO_F32 || dtype == DINO_F64, DINO_EINVAL, "%s: invalid dtype %d",
                name, dtype);
-  for (int k = 0; k < n_src; ++k) DINO_REQUIRE(src[k], DINO_EINVAL, "%s: NULL source %d", name, k);
+  for (int k = 0; k < n_src; ++k) {
+    DINO_REQUIRE(src[k], DINO_EINVAL, "%s: NULL source %d", name, k);
+    DINO_REQUIRE((flags[k] & ~(DINO_GATHER_IDENTITY | DINO_GATHER_SCALED)) == 0, DINO_EINVAL,
+                 "%s: invalid flags %d", name, flags[k]);
+    DINO_REQUIRE(!(flags[k] & DINO_GATHER_SCALED) ||
+                     (degree_scale && scale_len > 0 && n % scale_len == 0),
+                 DINO_EINVAL, "%s: a scaled source needs a scale whose length divides %lld",
+                 name, (long long)n);
+  }
   int64_t ib = 0, ie = n, off = 0, n_out = n;
   if (ranged) {
     int rc = check_range(name, n, im_begin, im_end, g_origin);
@@ -568,10 +640,10 @@ extern "C" int dino_imex_adjoint_gather(const void* const* src, const double* co
   if (n_out == 0) return DINO_OK;
   const int64_t trl = degrees ? field_size : 0;
   if (dtype == DINO_F64)
-    return launch_gather<double>(src, coef, identity, n_src, dt, out, n_out, off, ib, ie,
-                                 as_stream(stream), trl, degrees);
-  return launch_gather<float>(src, coef, identity, n_src, dt, out, n_out, off, ib, ie,
-                              as_stream(stream), trl, degrees);
+    return launch_gather<double>(src, coef, flags, n_src, dt, out, n_out, off, ib, ie,
+                                 as_stream(stream), trl, degrees, degree_scale, scale_len);
+  return launch_gather<float>(src, coef, flags, n_src, dt, out, n_out, off, ib, ie,
+                              as_stream(stream), trl, degrees, degree_scale, scale_len);
 }
 
 extern "C" int dino_imex_stage_adjoint(const void* out_cot, int n_ex, const double* a,

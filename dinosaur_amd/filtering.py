@@ -37,6 +37,9 @@ def _make_filter_fn(scaling: np.ndarray):
       return type(tree).from_flat(rescale(buf), *tree._flat[1:], sim_time=tree.sim_time)
     return spherical_harmonic.tree_map_over_nonscalars(rescale, tree)
 
+  # 1-D (per total wavenumber) scalings may be fused into the producing pass
+  apply._degree_scaling = scaling if scaling.ndim == 1 else None
+
   return apply
 
 

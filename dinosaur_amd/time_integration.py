@@ -213,24 +213,32 @@ def _code(t):
   return _lib.DINO_F64 if t.dtype == torch.float64 else _lib.DINO_F32
 
 
-def _fused_stage(ops, y0, dt, ex, im):
-  """y0 + dt sum(a f) + dt sum(b g) with compact implicit terms g (one pass)."""
+def _fused_stage(ops, y0, dt, ex, im, scale=None):
+  """y0 + dt sum(a f) + dt sum(b g) with compact implicit terms g (one pass),
+  times the per-degree `scale` (a step filter) when given."""
   from . import _lib  # pylint: disable=g-import-not-at-top
   out = torch.empty_like(y0)
   fs, gs = [x for _, x in ex], [x for _, x in im]
   stream = ctypes.c_void_p(torch.cuda.current_stream(y0.device).cuda_stream)
-  rc = _lib.lib().dino_imex_stage_ranged(
-      ctypes.c_void_p(y0.data_ptr()), len(fs), _ptrs(fs), _dbls([a for a, _ in ex]), len(gs),
-      _ptrs(gs), _dbls([b for b, _ in im]), float(dt), ctypes.c_void_p(out.data_ptr()),
-      y0.numel(), ops.ib, ops.ie, ops.go, ops.field_size, ops.degrees, _code(y0), stream)
-  _lib.check(rc, 'dino_imex_stage_ranged')
+  args = (ctypes.c_void_p(y0.data_ptr()), len(fs), _ptrs(fs), _dbls([a for a, _ in ex]),
+          len(gs), _ptrs(gs), _dbls([b for b, _ in im]), float(dt),
+          ctypes.c_void_p(out.data_ptr()), y0.numel(), ops.ib, ops.ie, ops.go, ops.field_size,
+          ops.degrees)
+  if scale is None:
+    rc = _lib.lib().dino_imex_stage_ranged(*args, _code(y0), stream)
+    _lib.check(rc, 'dino_imex_stage_ranged')
+  else:
+    rc = _lib.lib().dino_imex_stage_ranged_scaled(
+        *args, ctypes.c_void_p(scale.data_ptr()), scale.numel(), _code(y0), stream)
+    _lib.check(rc, 'dino_imex_stage_ranged_scaled')
   return out
 
 
-def _fused_gather(ops, sources, dt, like, ranged=False):
-  """One cotangent from its pending (source, coef, identity) contributions,
-  summed in list order in one pass (at most 8 sources: _fused_imex_step takes
-  tableaus of at most 7 stages); None when there are none."""
+def _fused_gather(ops, sources, dt, like, ranged=False, scale=None):
+  """One cotangent from its pending (source, coef, identity, scaled)
+  contributions, summed in list order in one pass (at most 8 sources:
+  _fused_imex_step takes tableaus of at most 7 stages); scaled sources are
+  multiplied by the per-degree `scale` first.  None when there are none."""
   from . import _lib  # pylint: disable=g-import-not-at-top
   if not sources:
     return None
@@ -238,20 +246,24 @@ def _fused_gather(ops, sources, dt, like, ranged=False):
     raise ValueError(f'{len(sources)} cotangent sources; at most 8 are gathered')
   out = ops.new_compact(like) if ranged else torch.empty_like(like)
   stream = ctypes.c_void_p(torch.cuda.current_stream(like.device).cuda_stream)
-  ident = (ctypes.c_int * len(sources))(*[1 if i else 0 for _, _, i in sources])
+  flags = (ctypes.c_int * len(sources))(*[
+      (_lib.GATHER_IDENTITY if i else 0) | (_lib.GATHER_SCALED if sc else 0)
+      for _, _, i, sc in sources])
   rc = _lib.lib().dino_imex_adjoint_gather(
-      _ptrs([t for t, _, _ in sources]), _dbls([c for _, c, _ in sources]), ident,
+      _ptrs([t for t, _, _, _ in sources]), _dbls([c for _, c, _, _ in sources]), flags,
       len(sources), float(dt), ctypes.c_void_p(out.data_ptr()), like.numel(),
-      1 if ranged else 0, ops.ib, ops.ie, ops.go, ops.field_size, ops.degrees, _code(like),
-      stream)
+      1 if ranged else 0, ops.ib, ops.ie, ops.go, ops.field_size, ops.degrees,
+      ctypes.c_void_p(scale.data_ptr() if scale is not None else 0),
+      scale.numel() if scale is not None else 0, _code(like), stream)
   _lib.check(rc, 'dino_imex_adjoint_gather')
   return out
 
 
-def _fused_forward(ops, tab, dt, y0, keep):
+def _fused_forward(ops, tab, dt, y0, keep, out_scale=None):
   """The stages of one IMEX step on flat buffers.  keep=True records each
   explicit evaluation's autograd graph (leaf input, output) for the reverse
-  sweep."""
+  sweep; out_scale, a per-degree step filter, scales the output in the
+  output combination's pass."""
   a_ex, a_im, b_ex, b_im = tab.a_ex, tab.a_im, tab.b_ex, tab.b_im
   n = len(b_ex)
   f, g, leaves = [None] * n, [None] * n, [None] * n
@@ -277,7 +289,7 @@ def _fused_forward(ops, tab, dt, y0, keep):
       g[i] = ops.terms(y)
   ex = [(b_ex[j], f[j]) for j in range(n) if b_ex[j]]
   im = [(b_im[j], g[j]) for j in range(n) if b_im[j]]
-  return _fused_stage(ops, y0, dt, ex, im), f, g, leaves
+  return _fused_stage(ops, y0, dt, ex, im, out_scale), f, g, leaves
 
 
 class _FusedImexStep(torch.autograd.Function):
@@ -294,9 +306,9 @@ class _FusedImexStep(torch.autograd.Function):
   """
 
   @staticmethod
-  def forward(ctx, ops, tab, dt, y0):
-    out, f, g, leaves = _fused_forward(ops, tab, dt, y0, keep=True)
-    ctx.ops, ctx.tab, ctx.dt = ops, tab, dt
+  def forward(ctx, ops, tab, dt, y0, out_scale):
+    out, f, g, leaves = _fused_forward(ops, tab, dt, y0, keep=True, out_scale=out_scale)
+    ctx.ops, ctx.tab, ctx.dt, ctx.out_scale = ops, tab, dt, out_scale
     ctx.f, ctx.leaves = f, leaves
     ctx.has_g = [x is not None for x in g]
     return out
@@ -307,16 +319,21 @@ class _FusedImexStep(torch.autograd.Function):
     a_ex, a_im, b_ex, b_im = tab.a_ex, tab.a_im, tab.b_ex, tab.b_im
     obar = obar.contiguous()
     n = len(b_ex)
+    sc = ctx.out_scale
     # Each cotangent is formed once, when it is first needed, from the list of
-    # its contributions (output cotangent, then the stage cotangents in
-    # reverse stage order): the same sums, in the same order, as writing it
-    # at the output combination and accumulating at every stage.
-    pend_f = {j: [(obar, b_ex[j], False)] for j in range(n) if b_ex[j] and f[j] is not None}
-    pend_g = {j: [(obar, b_im[j], False)] for j in range(n) if b_im[j] and ctx.has_g[j]}
-    pend_y0 = [(obar, 1.0, True)]
+    # its contributions (output cotangent -- through the output filter's
+    # per-degree scale, if any -- then the stage cotangents in reverse stage
+    # order): the same sums, in the same order, as writing it at the output
+    # combination and accumulating at every stage.
+    out_sc = sc is not None
+    pend_f = {j: [(obar, b_ex[j], False, out_sc)] for j in range(n)
+              if b_ex[j] and f[j] is not None}
+    pend_g = {j: [(obar, b_im[j], False, out_sc)] for j in range(n)
+              if b_im[j] and ctx.has_g[j]}
+    pend_y0 = [(obar, 1.0, True, out_sc)]
     for i in range(n - 1, 0, -1):
-      fbar = _fused_gather(ops, pend_f.pop(i, None), dt, obar)
-      gbar = _fused_gather(ops, pend_g.pop(i, None), dt, obar, ranged=True)
+      fbar = _fused_gather(ops, pend_f.pop(i, None), dt, obar, scale=sc)
+      gbar = _fused_gather(ops, pend_g.pop(i, None), dt, obar, ranged=True, scale=sc)
       ybar = None
       if f[i] is not None and fbar is not None:
         (ybar,) = torch.autograd.grad(f[i], leaves[i], fbar)
@@ -330,28 +347,29 @@ class _FusedImexStep(torch.autograd.Function):
         continue
       ybar = ybar.contiguous()
       ops.solve_adjoint_(ybar, dt * a_im[i - 1][i])
-      pend_y0.append((ybar, 1.0, True))
+      pend_y0.append((ybar, 1.0, True, False))
       for j in range(i):
         if a_ex[i - 1][j] and f[j] is not None:
-          pend_f.setdefault(j, []).append((ybar, a_ex[i - 1][j], False))
+          pend_f.setdefault(j, []).append((ybar, a_ex[i - 1][j], False, False))
         if a_im[i - 1][j] and ctx.has_g[j]:
-          pend_g.setdefault(j, []).append((ybar, a_im[i - 1][j], False))
-    fbar0 = _fused_gather(ops, pend_f.pop(0, None), dt, obar)
-    gbar0 = _fused_gather(ops, pend_g.pop(0, None), dt, obar, ranged=True)
+          pend_g.setdefault(j, []).append((ybar, a_im[i - 1][j], False, False))
+    fbar0 = _fused_gather(ops, pend_f.pop(0, None), dt, obar, scale=sc)
+    gbar0 = _fused_gather(ops, pend_g.pop(0, None), dt, obar, ranged=True, scale=sc)
     if f[0] is not None and fbar0 is not None:
       (gy0,) = torch.autograd.grad(f[0], leaves[0], fbar0)
-      pend_y0.append((gy0.contiguous(), 1.0, True))
+      pend_y0.append((gy0.contiguous(), 1.0, True, False))
     del fbar0
-    ybar0 = _fused_gather(ops, pend_y0, dt, obar)
+    ybar0 = _fused_gather(ops, pend_y0, dt, obar, scale=sc)
     if ctx.has_g[0] and gbar0 is not None:
       ops.terms_adjoint_acc(gbar0, ybar0)
     ctx.f = ctx.leaves = None
-    return None, None, None, ybar0
+    return None, None, None, ybar0, None
 
 
-def _fused_imex_step(ops, tab, dt, y0):
+def _fused_imex_step(ops, tab, dt, y0, out_scale=None):
   """One IMEX step on a flat state through the fused kernels, or None when
-  the tableau does not fit them (at most 4 terms per group, 7 stages)."""
+  the tableau does not fit them (at most 4 terms per group, 7 stages).
+  out_scale: per-degree step filter applied to the output in its pass."""
   n = len(tab.b_ex)
   rows = [tab.a_ex[i] for i in range(n - 1)] + [tab.b_ex]
   rows_im = [tab.a_im[i] for i in range(n - 1)] + [tab.b_im]
@@ -359,10 +377,10 @@ def _fused_imex_step(ops, tab, dt, y0):
     return None
   buf = y0.flat_buffer()
   if torch.is_grad_enabled() and buf.requires_grad:
-    out = _FusedImexStep.apply(ops, tab, dt, buf)
+    out = _FusedImexStep.apply(ops, tab, dt, buf, out_scale)
   else:
     with torch.no_grad():
-      out = _fused_forward(ops, tab, dt, buf, keep=False)[0]
+      out = _fused_forward(ops, tab, dt, buf, keep=False, out_scale=out_scale)[0]
   t0 = y0.sim_time
   st = None
   if t0 is not None:
@@ -383,16 +401,29 @@ def imex_runge_kutta(tableau: ImExButcherTableau, equation, time_step: float):
 
   hook = getattr(equation, '_flat_imex_ops', None)
 
-  def step_fn(y0):
-    # one flat buffer per state: every stage combination is a single pass
+  def fused(y0, filter_fn):
+    """(flat y0, fused step output or None)."""
     as_flat = getattr(y0, 'as_flat', None)
     if as_flat is not None:
       y0 = as_flat()
     ops = hook(y0) if (hook is not None and as_flat is not None) else None
-    if ops is not None:
-      out = _fused_imex_step(ops, tableau, dt, y0)
-      if out is not None:
-        return out
+    if ops is None:
+      return y0, None
+    scale = None
+    if filter_fn is not None:
+      scale = filter_fn._degree_scale_tensor(y0.flat_buffer())
+      if scale is None:
+        return y0, None
+    return y0, _fused_imex_step(ops, tableau, dt, y0, scale)
+
+  def step_fn(y0):
+    # one flat buffer per state: every stage combination is a single pass
+    y0, out = fused(y0, None)
+    if out is not None:
+      return out
+    return unfused(y0)
+
+  def unfused(y0):
     f = [None] * n
     g = [None] * n
     f[0] = F(y0)
@@ -410,6 +441,17 @@ def imex_runge_kutta(tableau: ImExButcherTableau, equation, time_step: float):
     im = [(b_im[j], g[j]) for j in range(n) if b_im[j]]
     return _stage(y0, dt, ex, im)
 
+  def with_filter(filter_fn):
+    """The step followed by `filter_fn` (a per-degree scaling step filter),
+    the scaling done in the fused output pass when the fused path runs."""
+    def filtered_step_fn(u):
+      y0, out = fused(u, filter_fn)
+      if out is not None:
+        return out
+      return filter_fn(u, step_fn(u))
+    return filtered_step_fn
+
+  step_fn._with_scaling_filter = with_filter
   return step_fn
 
 
@@ -430,6 +472,25 @@ def runge_kutta_step_filter(state_filter):
   def _filter(u, u_next):
     del u
     return state_filter(u_next)
+  # a per-degree scaling filter can run inside an IMEX step's output pass
+  scaling = getattr(state_filter, '_degree_scaling', None)
+  if scaling is not None:
+    cache = {}
+
+    def degree_scale_tensor(buf):
+      """The scaling as a tensor for a flat buffer whose last axis it spans
+      (rounded to the buffer's type, as the elementwise product rounds it),
+      or None."""
+      if buf.shape[-1] != scaling.shape[-1]:
+        return None
+      key = (buf.device, buf.dtype)
+      t = cache.get(key)
+      if t is None:
+        t = torch.as_tensor(scaling, device=buf.device).to(buf.dtype).contiguous()
+        cache[key] = t
+      return t
+
+    _filter._degree_scale_tensor = degree_scale_tensor
   return _filter
 
 
@@ -447,6 +508,13 @@ def horizontal_diffusion_step_filter(grid, dt: float, tau: float, order: int = 1
 
 
 def step_with_filters(step_fn, filters):
+  # one per-degree scaling filter after an IMEX step: fused into the step's
+  # output pass (same roundings: one product per element)
+  fuse = getattr(step_fn, '_with_scaling_filter', None)
+  if (fuse is not None and len(filters) == 1 and
+      getattr(filters[0], '_degree_scale_tensor', None) is not None):
+    return fuse(filters[0])
+
   def _step_fn(u):
     u_next = step_fn(u)
     for filter_fn in filters:

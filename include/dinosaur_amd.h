@@ -272,6 +272,20 @@ int dino_imex_stage_ranged(const void* y0, int n_ex, const void* const* f,
                            int64_t field_size, int degrees, int dtype,
                            void* stream);
 
+/* dino_imex_stage_ranged followed by a per-degree scaling of the result,
+ * out[i] = y[i] * degree_scale[i % scale_len] rounded once (scale_len
+ * elements of the data type, dividing n): an IMEX step's output combination
+ * with the exponential or diffusion step filter (filtering.py:38-100 via
+ * time_integration.py:427-486, one elementwise product after the step in
+ * the reference) in the same pass. */
+int dino_imex_stage_ranged_scaled(const void* y0, int n_ex, const void* const* f,
+                                  const double* a, int n_im, const void* const* g,
+                                  const double* b, double dt, void* out, int64_t n,
+                                  int64_t im_begin, int64_t im_end,
+                                  int64_t g_origin, int64_t field_size,
+                                  int degrees, const void* degree_scale,
+                                  int64_t scale_len, int dtype, void* stream);
+
 /* Cotangent targets of one stage combination, written in one pass over the
  * output cotangent gbar: for each target t,
  *   v = gbar                       (kind has DINO_TARGET_IDENTITY: y0),
@@ -300,8 +314,12 @@ int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
 
 /* One cotangent of a reverse IMEX sweep gathered from all its sources in one
  * pass: out = v_0 + v_1 + ... + v_{n_src-1} in source order, with
- *   v_k = src[k]                   (identity[k] != 0: a y0 contribution),
- *   v_k = coef[k] * (dt * src[k])  (otherwise: a stage coefficient),
+ *   x_k = src[k] * degree_scale[i % scale_len]  (flags[k] has
+ *         DINO_GATHER_SCALED: the cotangent of a per-degree filter applied
+ *         to the step output), else x_k = src[k];
+ *   v_k = x_k                      (flags[k] has DINO_GATHER_IDENTITY: a y0
+ *         contribution),
+ *   v_k = coef[k] * (dt * x_k)     (otherwise: a stage coefficient);
  * the same roundings and order as one dino_imex_stage_adjoint_targets pass
  * writing v_0 followed by passes accumulating v_1, v_2, ... -- so a sweep
  * may wait until a term's cotangent is needed and form it once instead of
@@ -310,13 +328,17 @@ int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
  * ranged != 0 writes a ranged implicit cotangent (indexing as in
  * dino_imex_stage_ranged, nothing written outside [im_begin, im_end)).
  * field_size / degrees as for dino_imex_stage_ranged: zero where the kernel
- * skips the triangle.  1 to 8 sources; out must not overlap them. */
+ * skips the triangle.  1 to 8 sources; out must not overlap them;
+ * degree_scale (scale_len elements, scale_len dividing n) may be NULL when
+ * no source is scaled. */
+enum { DINO_GATHER_IDENTITY = 1, DINO_GATHER_SCALED = 2 };
 int dino_imex_adjoint_gather(const void* const* src, const double* coef,
-                             const int* identity, int n_src, double dt,
+                             const int* flags, int n_src, double dt,
                              void* out, int64_t n, int ranged,
                              int64_t im_begin, int64_t im_end,
                              int64_t g_origin, int64_t field_size,
-                             int degrees, int dtype, void* stream);
+                             int degrees, const void* degree_scale,
+                             int64_t scale_len, int dtype, void* stream);
 
 /* Fused nodal nonlinear terms of PrimitiveEquationsSigma.explicit_terms.
  * Replaces compute_diagnostic_state_sigma and the nodal parts of

@@ -335,9 +335,10 @@ def test_adjoint_gather_matches_target_passes(dtype, gpu_device):
   roundings and order of one dino_imex_stage_adjoint_targets pass writing the
   first contribution followed by passes accumulating the others: bitwise
   equal for full and ranged (implicit-range) outputs, identity and scaled
-  sources, vector and scalar paths (including the skipped parts below the
-  triangle), and nothing written outside the range (T21 Real-layout fields,
-  4 levels + tracer)."""
+  sources, sources scaled per degree (as the elementwise product would
+  scale them), vector and scalar paths (including the skipped parts below
+  the triangle), and nothing written outside the range (T21 Real-layout
+  fields, 4 levels + tracer)."""
   import ctypes
   from dinosaur_amd import _lib
   print(f'[test] adjoint gather {dtype}', flush=True)
@@ -355,24 +356,30 @@ def test_adjoint_gather_matches_target_passes(dtype, gpu_device):
   P = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])  # noqa: E731
   D = lambda cs: (ctypes.c_double * len(cs))(*cs)  # noqa: E731
   I = lambda xs: (ctypes.c_int * len(xs))(*xs)  # noqa: E731
+  lscale = torch.rand(L, generator=gen, device=gpu_device, dtype=dtype) + 0.5
   for shift in (0, 1):  # 1: buffers off 16-byte alignment, the scalar path
     srcs = [torch.randn(n + shift, generator=gen, device=gpu_device, dtype=dtype)[shift:]
             for _ in range(4)]
     for ranged in (0, 1):
       size = ie - go if ranged else n
-      for ident in ([1, 1, 1, 1], [0, 0, 0, 0], [1, 0, 1, 0]):
+      for ident, scaled in (([1, 1, 1, 1], [0, 0, 0, 0]), ([0, 0, 0, 0], [0, 0, 0, 0]),
+                            ([1, 0, 1, 0], [0, 0, 0, 0]), ([1, 0, 0, 1], [1, 1, 0, 0])):
         nan = lambda: torch.full((size + shift,), float('nan'), device=gpu_device,  # noqa: E731
                                  dtype=dtype)[shift:]
         ref, out = nan(), nan()
         for k, src in enumerate(srcs):
           kind = ((_lib.TARGET_IDENTITY if ident[k] else 0) |
                   (_lib.TARGET_RANGED if ranged else 0) | (_lib.TARGET_ACCUMULATE if k else 0))
+          # a scaled source is the elementwise product with the per-degree scale
+          x = (src.view(-1, L) * lscale).reshape(-1) if scaled[k] else src
           _lib.check(lib.dino_imex_stage_adjoint_targets(
-              ctypes.c_void_p(src.data_ptr()), 1, P([ref]), D([coef[k]]), I([kind]), dt, n, ib,
+              ctypes.c_void_p(x.data_ptr()), 1, P([ref]), D([coef[k]]), I([kind]), dt, n, ib,
               ie, go, fs, L, code, stream), 'targets')
+        flags = [(_lib.GATHER_IDENTITY if i else 0) | (_lib.GATHER_SCALED if c else 0)
+                 for i, c in zip(ident, scaled)]
         _lib.check(lib.dino_imex_adjoint_gather(
-            P(srcs), D(coef), I(ident), 4, dt, ctypes.c_void_p(out.data_ptr()), n, ranged, ib,
-            ie, go, fs, L, code, stream), 'gather')
+            P(srcs), D(coef), I(flags), 4, dt, ctypes.c_void_p(out.data_ptr()), n, ranged, ib,
+            ie, go, fs, L, ctypes.c_void_p(lscale.data_ptr()), L, code, stream), 'gather')
         if ranged:
           pad = ib - go
           assert torch.isnan(out[:pad]).all() and torch.isnan(ref[:pad]).all()
@@ -382,8 +389,56 @@ def test_adjoint_gather_matches_target_passes(dtype, gpu_device):
   for bad in (0, 9):
     rc = lib.dino_imex_adjoint_gather(P(srcs), D(coef * 3), I([0] * 12), bad, dt,
                                       ctypes.c_void_p(srcs[0].data_ptr()), n, 0, ib, ie, go, fs,
-                                      L, code, stream)
+                                      L, None, 0, code, stream)
     assert rc == _lib.DINO_EINVAL
+  # a scaled source without a scale, and an unknown flag
+  for flags in ([2, 0, 0, 0], [4, 0, 0, 0]):
+    rc = lib.dino_imex_adjoint_gather(P(srcs), D(coef), I(flags), 4, dt,
+                                      ctypes.c_void_p(srcs[0].data_ptr()), n, 0, ib, ie, go, fs,
+                                      L, None, 0, code, stream)
+    assert rc == _lib.DINO_EINVAL
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
+def test_step_filter_fused_into_output_pass(dtype, gpu_device):
+  """step_with_filters(imex step, [exponential filter]) applies the filter's
+  per-degree scaling in the step's output pass (and through the gather in the
+  reverse sweep): forward state and gradients bitwise equal to the step
+  followed by the separate elementwise filter (moist T21 L6, 2 members)."""
+  print(f'[test] fused step filter {dtype}', flush=True)
+  grid = sh.Grid.T21()
+  K = 6
+  coords = coordinate_systems.CoordinateSystem(grid, sigma_coordinates.SigmaCoordinates.equidistant(K))
+  oro = np.zeros(grid.modal_shape)
+  eq = pe.PrimitiveEquationsSigma(np.full(K, 250.), oro, coords, units.SimUnits.from_si(),
+                                  humidity_key='q')
+  dt = 0.02
+  base = ti.imex_rk_sil3(eq, dt)
+  filt = ti.exponential_step_filter(grid, dt)
+  fused = ti.step_with_filters(base, [filt])
+  assert fused is not base and getattr(base, '_with_scaling_filter', None) is not None
+  separate = lambda u: filt(u, base(u))  # noqa: E731
+  st = _random_state(coords, ['q'], gpu_device, 21, dtype)
+  st = pe.State(*(torch.stack([getattr(st, f), 0.5 * getattr(st, f)]) for f in FIELDS),
+                tracers={'q': torch.stack([st.tracers['q'], 0.5 * st.tracers['q']])})
+  results = []
+  for fn in (fused, separate):
+    leaves = {f: getattr(st, f).clone().requires_grad_(True) for f in FIELDS}
+    trl = {k: v.clone().requires_grad_(True) for k, v in st.tracers.items()}
+    y = fn(fn(pe.State(**leaves, tracers=trl)))
+    loss = (y.temperature_variation ** 2).sum() + (y.vorticity * y.divergence).sum() + \
+        (y.tracers['q'] * y.log_surface_pressure).sum()
+    grads = torch.autograd.grad(loss, list(leaves.values()) + list(trl.values()))
+    results.append((y, grads))
+  (ya, ga), (yb, gb) = results
+  for f in FIELDS:
+    assert torch.equal(getattr(ya, f), getattr(yb, f)), f
+  assert torch.equal(ya.tracers['q'], yb.tracers['q'])
+  for a, b in zip(ga, gb):
+    assert torch.equal(a, b)
+  # the filter does act: the filtered step differs from the bare step
+  with torch.no_grad():
+    assert not torch.equal(fused(st).vorticity, base(st).vorticity)
 
 
 @pytest.mark.parametrize('keep', [0, 1, 'auto'])

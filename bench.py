@@ -505,7 +505,7 @@ def main():
   torch.cuda.empty_cache()
   recompute_ms = None
   if not args.forward_only and rollout.get('kept'):
-    rwall, _ = time_leg(torch, step, state, coords, targets, args.steps, 0, False, 0)
+    rwall, _ = time_leg(torch, step, state, coords, targets, args.steps, 1, False, 0)
     recompute_ms = rwall / args.steps * 1e3
     print(f'[bench] chosen, every step recomputed: {recompute_ms:.1f} ms/step',
           file=sys.stderr, flush=True)

@@ -16,8 +16,9 @@
 //            operand as they stand: f64 accumulator register i of row tile it
 //            holds row 16 it + 4 i + kg, which is row 4 s + kg of k-step
 //            s = 4 it + i.
-// Matrix operands come from fragment-ordered tables built at create time (one
-// 8-byte load per lane per MFMA, 512 contiguous bytes per wave).  Levels are
+// Matrix operands come from fragment-ordered tables built at create time,
+// copied into LDS once per 8-wave block (one 8-byte LDS read per lane per
+// MFMA, 512 contiguous bytes per wave).  Levels are
 // padded to KP = 4 ceil(K / 4); the H V block starts at row KP, so the
 // temperature fragments a lane loaded for stage 1 are the ones its stage-2
 // output rows need.
@@ -31,6 +32,8 @@ namespace dino {
 namespace {
 
 typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWavesPerBlock = 8;  // waves sharing one LDS copy of the tables
 
 __device__ __forceinline__ doublex4 mma64(double a, double b, doublex4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -85,14 +88,24 @@ __device__ __forceinline__ void zero_columns(T* div_out, T* temp_out, T* lnps_ou
 // In place is allowed (outputs equal to inputs): a wave reads every level of
 // its 16 columns before it writes any, and no other wave touches them.
 template <typename T, int NS, bool TRANS>
-__global__ __launch_bounds__(256) void solve_mfma_kernel(
+__global__ __launch_bounds__(64 * kWavesPerBlock) void solve_mfma_kernel(
     const T* div, const T* temp, const T* lnps, T* div_out, T* temp_out, T* lnps_out,
-    const double* __restrict__ t1a, const double* __restrict__ t1b,
-    const double* __restrict__ t2, const double* __restrict__ wvec,
+    const double* __restrict__ t1a_g, const double* __restrict__ t1b_g,
+    const double* __restrict__ t2_g, const double* __restrict__ wvec,
     const double* __restrict__ dvec, const double* __restrict__ lam, int K, int L,
     int64_t cols, int64_t members, double eta, int tri, int inplace) {
   constexpr bool ACCUM = false;
   using S = Shape<NS>;
+  // the block's waves share the operand tables through LDS (one L2 read per
+  // block instead of one per wave and MFMA)
+  constexpr int N1 = S::T1 * NS * 64, N2 = S::T2 * S::S2 * 64;
+  __shared__ double t1a[N1], t1b[N1], t2[N2];
+  for (int i = threadIdx.x; i < N1; i += blockDim.x) {
+    t1a[i] = t1a_g[i];
+    t1b[i] = t1b_g[i];
+  }
+  for (int i = threadIdx.x; i < N2; i += blockDim.x) t2[i] = t2_g[i];
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tiles_per_member = (cols + 15) / 16;
@@ -179,13 +192,18 @@ __global__ __launch_bounds__(256) void solve_mfma_kernel(
 // ACC: add the result to the outputs instead of overwriting them.  In place
 // is allowed, as for the solve.
 template <typename T, int NS, bool TRANS, bool ACC>
-__global__ __launch_bounds__(256) void terms_mfma_kernel(
+__global__ __launch_bounds__(64 * kWavesPerBlock) void terms_mfma_kernel(
     const T* div, const T* temp, const T* lnps, T* div_out, T* temp_out, T* lnps_out,
-    const double* __restrict__ tg, const double* __restrict__ th,
+    const double* __restrict__ tg_g, const double* __restrict__ th_g,
     const double* __restrict__ rtref, const double* __restrict__ lam, int K, int L,
     int64_t cols, int64_t members, int tri, int inplace) {
   constexpr bool ACCUM = ACC;  // adding the zero result changes nothing
   using S = Shape<NS>;
+  constexpr int NG = S::T1 * NS * 64, NH = S::TT * NS * 64;
+  __shared__ double tg[NG], th[NH];
+  for (int i = threadIdx.x; i < NG; i += blockDim.x) tg[i] = tg_g[i];
+  for (int i = threadIdx.x; i < NH; i += blockDim.x) th[i] = th_g[i];
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tiles_per_member = (cols + 15) / 16;
@@ -359,7 +377,8 @@ static int launch_ns(bool solve, bool adjoint, bool acc, const dino_implicit* f,
   const int64_t cols = (int64_t)f->R * f->L;
   const int64_t waves = (cols + 15) / 16 * members;
   if (waves == 0) return DINO_OK;
-  const unsigned blocks = (unsigned)((waves + 3) / 4);
+  const unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+  const unsigned threads = 64 * kWavesPerBlock;
   const T* x0 = (const T*)div;
   const T* x1 = (const T*)temp;
   const T* x2 = (const T*)lnps;
@@ -370,23 +389,23 @@ static int launch_ns(bool solve, bool adjoint, bool acc, const dino_implicit* f,
   const int tri = (f->R % 2 == 1) ? 1 : 0;  // Real layout (odd row count)
   const int inplace = (d_o == div && t_o == temp && l_o == lnps) ? 1 : 0;
   if (solve && !adjoint) {
-    solve_mfma_kernel<T, NS, false><<<blocks, 256, 0, s>>>(
+    solve_mfma_kernel<T, NS, false><<<blocks, threads, 0, s>>>(
         x0, x1, x2, y0, y1, y2, t[0], t[1], t[2], t[3], t[4], f->lam, f->K, f->L, cols,
         members, f->eta, tri, inplace);
   } else if (solve) {
-    solve_mfma_kernel<T, NS, true><<<blocks, 256, 0, s>>>(
+    solve_mfma_kernel<T, NS, true><<<blocks, threads, 0, s>>>(
         x0, x1, x2, y0, y1, y2, t[8], t[9], t[10], t[11], t[4], f->lam, f->K, f->L, cols,
         members, f->eta, tri, inplace);
   } else if (!adjoint) {
-    terms_mfma_kernel<T, NS, false, false><<<blocks, 256, 0, s>>>(
+    terms_mfma_kernel<T, NS, false, false><<<blocks, threads, 0, s>>>(
         x0, x1, x2, y0, y1, y2, t[5], t[6], t[7], f->lam, f->K, f->L, cols, members, tri,
         inplace);
   } else if (!acc) {
-    terms_mfma_kernel<T, NS, true, false><<<blocks, 256, 0, s>>>(
+    terms_mfma_kernel<T, NS, true, false><<<blocks, threads, 0, s>>>(
         x0, x1, x2, y0, y1, y2, t[12], t[13], t[14], f->lam, f->K, f->L, cols, members, tri,
         inplace);
   } else {
-    terms_mfma_kernel<T, NS, true, true><<<blocks, 256, 0, s>>>(
+    terms_mfma_kernel<T, NS, true, true><<<blocks, threads, 0, s>>>(
         x0, x1, x2, y0, y1, y2, t[12], t[13], t[14], f->lam, f->K, f->L, cols, members, tri,
         inplace);
   }

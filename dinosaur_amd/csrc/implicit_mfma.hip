@@ -88,7 +88,8 @@ __device__ __forceinline__ void zero_columns(T* div_out, T* temp_out, T* lnps_ou
 // In place is allowed (outputs equal to inputs): a wave reads every level of
 // its 16 columns before it writes any, and no other wave touches them.
 template <typename T, int NS, bool TRANS>
-__global__ __launch_bounds__(64 * kWavesPerBlock) void solve_mfma_kernel(
+__global__ __launch_bounds__(64 * kWavesPerBlock, (!TRANS && sizeof(T) == 4 && NS <= 10) ? 4 : 1)
+void solve_mfma_kernel(
     const T* div, const T* temp, const T* lnps, T* div_out, T* temp_out, T* lnps_out,
     const double* __restrict__ t1a_g, const double* __restrict__ t1b_g,
     const double* __restrict__ t2_g, const double* __restrict__ wvec,

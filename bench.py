@@ -240,9 +240,11 @@ def accuracy(torch, precision, members=2, steps=1, impl=None, fused_nodal=True):
 
 
 def time_leg(torch, step, state, coords, targets, steps, warmup, forward_only=False,
-             keep_graphs=0, report=None):
+             keep_graphs=0, report=None, warm_len=1):
+  # warm_len: steps per warm-up rollout (a rollout that keeps every graph
+  # needs a full-length warm-up, or its first allocations land in the timing)
   for _ in range(warmup):
-    rollout_fwd_adj(torch, step, state, coords, targets, 1, forward_only, keep_graphs)
+    rollout_fwd_adj(torch, step, state, coords, targets, warm_len, forward_only, keep_graphs)
   _sync(torch)
   t0 = time.perf_counter()
   out = rollout_fwd_adj(torch, step, state, coords, targets, steps, forward_only, keep_graphs,
@@ -511,7 +513,7 @@ def main():
     other_keep = 0 if rollout.get('kept') else 'auto'
     orep = {}
     owall, _ = time_leg(torch, step, state, coords, targets, args.steps, 1, False, other_keep,
-                        orep)
+                        orep, warm_len=args.steps)
     other = {'keep_graphs': other_keep, 'ms_per_step': owall / args.steps * 1e3,
              'graphs_kept': orep.get('kept'),
              'member_sim_days_per_s': members * args.steps * days / owall}

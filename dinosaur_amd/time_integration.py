@@ -11,6 +11,7 @@ import ctypes
 import dataclasses
 import functools
 import math
+import threading
 from typing import Any, Callable, Sequence
 
 import torch
@@ -259,14 +260,38 @@ def _fused_gather(ops, sources, dt, like, ranged=False, scale=None):
   return out
 
 
-def _fused_forward(ops, tab, dt, y0, keep, out_scale=None):
+def _uses(tab, n):
+  """(f_used, g_used): whether stage i's explicit / implicit tendency enters a
+  later stage or the output (the condition for computing it at all)."""
+  a_ex, a_im, b_ex, b_im = tab.a_ex, tab.a_im, tab.b_ex, tab.b_im
+  f_used = [bool(any(a_ex[j][i] for j in range(max(i, 0), n - 1)) or b_ex[i]) for i in range(n)]
+  g_used = [bool(any(a_im[j][i] for j in range(max(i, 0), n - 1)) or b_im[i]) for i in range(n)]
+  return f_used, g_used
+
+
+def _fused_forward(ops, tab, dt, y0, keep, out_scale=None, need_output=True):
   """The stages of one IMEX step on flat buffers.  keep=True records each
   explicit evaluation's autograd graph (leaf input, output) for the reverse
   sweep; out_scale, a per-degree step filter, scales the output in the
-  output combination's pass."""
+  output combination's pass.  need_output=False (a recomputation whose
+  output value nobody reads) computes only what the reverse sweep reads --
+  the recorded explicit evaluations and the stage values and implicit terms
+  they depend on -- and returns an uninitialised output."""
   a_ex, a_im, b_ex, b_im = tab.a_ex, tab.a_im, tab.b_ex, tab.b_im
   n = len(b_ex)
   f, g, leaves = [None] * n, [None] * n, [None] * n
+  f_used, g_used = _uses(tab, n)
+  # stage i is needed if its explicit tendency is recorded, or its implicit
+  # tendency feeds a needed later stage (the output does not count here)
+  need_f, need_g = list(f_used), list(g_used)
+  if not need_output:
+    need_stage = [False] * n
+    need_g = [False] * n
+    for i in range(n - 1, -1, -1):
+      need_stage[i] = f_used[i] or any(
+          need_stage[j + 1] and a_im[j][i] for j in range(i, n - 1))
+      need_g[i] = g_used[i] and any(need_stage[j + 1] and a_im[j][i] for j in range(i, n - 1))
+    need_f = [f_used[i] and need_stage[i] for i in range(n)]
 
   def explicit(i, y):
     if keep:
@@ -277,16 +302,21 @@ def _fused_forward(ops, tab, dt, y0, keep, out_scale=None):
       f[i] = ops.explicit(y)
 
   explicit(0, y0)
-  g[0] = ops.terms(y0)
+  if need_g[0]:
+    g[0] = ops.terms(y0)
   for i in range(1, n):
+    if not (need_f[i] or need_g[i]):
+      continue
     ex = [(a_ex[i - 1][j], f[j]) for j in range(i) if a_ex[i - 1][j]]
     im = [(a_im[i - 1][j], g[j]) for j in range(i) if a_im[i - 1][j]]
     y = _fused_stage(ops, y0, dt, ex, im)
     ops.solve_(y, dt * a_im[i - 1][i])
-    if any(a_ex[j][i] for j in range(i, n - 1)) or b_ex[i]:
+    if need_f[i]:
       explicit(i, y)
-    if any(a_im[j][i] for j in range(i, n - 1)) or b_im[i]:
+    if need_g[i]:
       g[i] = ops.terms(y)
+  if not need_output:
+    return torch.empty_like(y0), f, g, leaves
   ex = [(b_ex[j], f[j]) for j in range(n) if b_ex[j]]
   im = [(b_im[j], g[j]) for j in range(n) if b_im[j]]
   return _fused_stage(ops, y0, dt, ex, im, out_scale), f, g, leaves
@@ -306,11 +336,14 @@ class _FusedImexStep(torch.autograd.Function):
   """
 
   @staticmethod
-  def forward(ctx, ops, tab, dt, y0, out_scale):
-    out, f, g, leaves = _fused_forward(ops, tab, dt, y0, keep=True, out_scale=out_scale)
+  def forward(ctx, ops, tab, dt, y0, out_scale, need_output):
+    out, f, g, leaves = _fused_forward(ops, tab, dt, y0, keep=True, out_scale=out_scale,
+                                       need_output=need_output)
     ctx.ops, ctx.tab, ctx.dt, ctx.out_scale = ops, tab, dt, out_scale
     ctx.f, ctx.leaves = f, leaves
-    ctx.has_g = [x is not None for x in g]
+    # which implicit tendencies the step's formula contains (their adjoints
+    # need no forward values, so they apply whether or not g was computed)
+    ctx.has_g = [i == 0 or u for i, u in enumerate(_uses(tab, len(tab.b_ex))[1])]
     return out
 
   @staticmethod
@@ -363,7 +396,12 @@ class _FusedImexStep(torch.autograd.Function):
     if ctx.has_g[0] and gbar0 is not None:
       ops.terms_adjoint_acc(gbar0, ybar0)
     ctx.f = ctx.leaves = None
-    return None, None, None, ybar0, None
+    return None, None, None, ybar0, None, None
+
+
+# Set while a checkpoint recomputation runs the last step of its segment,
+# whose output value nobody reads.
+_recompute = threading.local()
 
 
 def _fused_imex_step(ops, tab, dt, y0, out_scale=None):
@@ -377,7 +415,8 @@ def _fused_imex_step(ops, tab, dt, y0, out_scale=None):
     return None
   buf = y0.flat_buffer()
   if torch.is_grad_enabled() and buf.requires_grad:
-    out = _FusedImexStep.apply(ops, tab, dt, buf, out_scale)
+    need = not getattr(_recompute, 'output_unread', False)
+    out = _FusedImexStep.apply(ops, tab, dt, buf, out_scale, need)
   else:
     with torch.no_grad():
       out = _fused_forward(ops, tab, dt, buf, keep=False, out_scale=out_scale)[0]
@@ -449,9 +488,11 @@ def imex_runge_kutta(tableau: ImExButcherTableau, equation, time_step: float):
       if out is not None:
         return out
       return filter_fn(u, step_fn(u))
+    filtered_step_fn._output_from_fused_step = True
     return filtered_step_fn
 
   step_fn._with_scaling_filter = with_filter
+  step_fn._output_from_fused_step = True
   return step_fn
 
 
@@ -647,12 +688,23 @@ def checkpointed_rollout(step_fn, x, steps: int, checkpoint_every: int, keep_gra
   leaves, rebuild = _flatten_state(x)
   structure = {}
 
-  def segment(rebuild_in, *ls):
+  def segment(rebuild_in, *ls, output_unread=False):
     # rebuild_in is bound per segment, so the recomputation in the backward
     # pass rebuilds the same structure as the forward call did
     y = rebuild_in(list(ls))
-    for _ in range(checkpoint_every):
-      y = step_fn(y)
+    for i in range(checkpoint_every):
+      # The recomputation's last output is only a root for differentiation:
+      # a step that is exactly a fused IMEX step may skip the work whose
+      # values the reverse sweep does not read.
+      if (output_unread and i == checkpoint_every - 1 and
+          getattr(step_fn, '_output_from_fused_step', False)):
+        _recompute.output_unread = True
+        try:
+          y = step_fn(y)
+        finally:
+          _recompute.output_unread = False
+      else:
+        y = step_fn(y)
     out, structure['rebuild'] = _flatten_state(y)
     return tuple(out)
 
@@ -698,7 +750,9 @@ class _CheckpointSegment(torch.autograd.Function):
   re-runs it from the saved input with gradients enabled and differentiates
   that recomputation.  Steps that record their own graphs when gradients are
   enabled (the fused IMEX step) therefore record them once per segment, in
-  the backward pass, and never during the first forward."""
+  the backward pass, and never during the first forward.  `fn` takes
+  output_unread=True in the recomputation: the values of its outputs are
+  never read there (the cotangents arrive from outside)."""
 
   @staticmethod
   def forward(ctx, fn, *leaves):
@@ -715,7 +769,7 @@ class _CheckpointSegment(torch.autograd.Function):
     saved = ctx.saved_tensors
     inputs = [t.detach().requires_grad_(t.requires_grad) for t in saved]
     with torch.enable_grad():
-      outs = ctx.fn(*inputs)
+      outs = ctx.fn(*inputs, output_unread=True)
     pairs = [(o, g) for o, g in zip(outs, gouts) if g is not None and o.requires_grad]
     want = [t for t in inputs if t.requires_grad]
     grads = torch.autograd.grad([o for o, _ in pairs], want, [g for _, g in pairs],

@@ -105,6 +105,18 @@ class StateShapeError(Exception):
   """Exceptions for unexpected state shapes (primitive_equations.py:87-88)."""
 
 
+
+# Set (per thread) while a recomputation evaluates terms whose values nobody
+# reads (only their recorded graph is differentiated): autograd Functions
+# whose backward reads neither their output nor anything the skipped kernel
+# computes may then return an uninitialised output without launching it.
+values_unread = threading.local()
+
+
+def skip_values() -> bool:
+  return getattr(values_unread, 'active', False)
+
+
 def lib() -> ctypes.CDLL:
   """Loads the HIP library (raises ImportError if it was not built)."""
   global _lib

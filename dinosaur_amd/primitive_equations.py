@@ -443,10 +443,11 @@ class _NodalFused(torch.autograd.Function):
     out = torch.empty((n_out,) + tuple(nodal.shape[1:]), dtype=nodal.dtype, device=nodal.device)
     code = _lib.DINO_F64 if nodal.dtype == torch.float64 else _lib.DINO_F32
     stream = ctypes.c_void_p(torch.cuda.current_stream(nodal.device).cuda_stream)
-    with torch.cuda.device(nodal.device):
-      rc = lib.dino_nodal_explicit(h.handle, ctypes.c_void_p(nodal.data_ptr()),
-                                   ctypes.c_void_p(out.data_ptr()), members, code, stream)
-    _lib.check(rc, 'dino_nodal_explicit')
+    if not _lib.skip_values():  # the backward reads only the saved inputs
+      with torch.cuda.device(nodal.device):
+        rc = lib.dino_nodal_explicit(h.handle, ctypes.c_void_p(nodal.data_ptr()),
+                                     ctypes.c_void_p(out.data_ptr()), members, code, stream)
+      _lib.check(rc, 'dino_nodal_explicit')
     ctx.save_for_backward(nodal)
     ctx.h, ctx.members, ctx.code = h, members, code
     return out
@@ -518,7 +519,8 @@ class _ModalEpilogue(torch.autograd.Function):
     x = x.contiguous()
     n = _lib.lib().dino_modal_fields(h.handle, members, 0)
     out = torch.empty((n,) + tuple(x.shape[1:]), dtype=x.dtype, device=x.device)
-    _launch('dino_modal_epilogue', h, x, orography, out, members, x)
+    if not _lib.skip_values():  # linear: the backward reads nothing saved
+      _launch('dino_modal_epilogue', h, x, orography, out, members, x)
     ctx.h, ctx.members, ctx.shape = h, members, tuple(x.shape)
     return out
 

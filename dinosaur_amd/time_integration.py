@@ -292,12 +292,26 @@ def _fused_forward(ops, tab, dt, y0, keep, out_scale=None, need_output=True):
           need_stage[j + 1] and a_im[j][i] for j in range(i, n - 1))
       need_g[i] = g_used[i] and any(need_stage[j + 1] and a_im[j][i] for j in range(i, n - 1))
     need_f = [f_used[i] and need_stage[i] for i in range(n)]
+    # a recorded explicit tendency that no needed stage combines: only its
+    # graph is differentiated, so its value need not be computed
+    graph_only = [need_f[i] and not any(need_stage[j + 1] and a_ex[j][i]
+                                        for j in range(i, n - 1)) for i in range(n)]
+  else:
+    graph_only = [False] * n
 
   def explicit(i, y):
     if keep:
       leaves[i] = y.detach().requires_grad_(True)
       with torch.enable_grad():
-        f[i] = ops.explicit(leaves[i])
+        if graph_only[i]:
+          from . import _lib  # pylint: disable=g-import-not-at-top
+          _lib.values_unread.active = True
+          try:
+            f[i] = ops.explicit(leaves[i])
+          finally:
+            _lib.values_unread.active = False
+        else:
+          f[i] = ops.explicit(leaves[i])
     else:
       f[i] = ops.explicit(y)
 

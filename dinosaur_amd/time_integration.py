@@ -235,6 +235,27 @@ def _fused_stage(ops, y0, dt, ex, im, scale=None):
   return out
 
 
+def _fused_stage_implicit_range(ops, y0, dt, ex, im):
+  """The stage combination over the implicit range [ib, ie) only -- all that
+  the implicit solve and terms read -- for a stage value that feeds nothing
+  else; the rest of the returned buffer is left uninitialised."""
+  from . import _lib  # pylint: disable=g-import-not-at-top
+  ib, ie, go = ops.ib, ops.ie, ops.go
+  out = torch.empty_like(y0)
+  sub = lambda t: t.view(-1)[ib:ie]  # noqa: E731
+  fs = [sub(x) for _, x in ex]
+  gs = [x[ib - go:ie - go] for _, x in im]
+  stream = ctypes.c_void_p(torch.cuda.current_stream(y0.device).cuda_stream)
+  n = ie - ib
+  rc = _lib.lib().dino_imex_stage_ranged(
+      ctypes.c_void_p(sub(y0).data_ptr()), len(fs), _ptrs(fs), _dbls([a for a, _ in ex]),
+      len(gs), _ptrs(gs), _dbls([b for b, _ in im]), float(dt),
+      ctypes.c_void_p(sub(out).data_ptr()), n, 0, n, 0, ops.field_size, ops.degrees,
+      _code(y0), stream)
+  _lib.check(rc, 'dino_imex_stage_ranged')
+  return out
+
+
 def _fused_gather(ops, sources, dt, like, ranged=False, scale=None):
   """One cotangent from its pending (source, coef, identity, scaled)
   contributions, summed in list order in one pass (at most 8 sources:
@@ -323,7 +344,10 @@ def _fused_forward(ops, tab, dt, y0, keep, out_scale=None, need_output=True):
       continue
     ex = [(a_ex[i - 1][j], f[j]) for j in range(i) if a_ex[i - 1][j]]
     im = [(a_im[i - 1][j], g[j]) for j in range(i) if a_im[i - 1][j]]
-    y = _fused_stage(ops, y0, dt, ex, im)
+    if need_f[i]:
+      y = _fused_stage(ops, y0, dt, ex, im)
+    else:  # only the implicit terms read this stage value
+      y = _fused_stage_implicit_range(ops, y0, dt, ex, im)
     ops.solve_(y, dt * a_im[i - 1][i])
     if need_f[i]:
       explicit(i, y)

@@ -24,7 +24,7 @@ SCHED_AUTO, SCHED_ROWS, SCHED_TILED, SCHED_TILED_NARROW = 0, 1, 2, 3
 SCHED_PERSISTENT, SCHED_PER_TILE = 1, 2
 SCHED_MPAIRED = 0x100
 TARGET_ACCUMULATE, TARGET_IDENTITY, TARGET_RANGED = 1, 2, 4
-GATHER_IDENTITY, GATHER_SCALED = 1, 2
+GATHER_IDENTITY, GATHER_SCALED, GATHER_IMPLICIT_RANGE = 1, 2, 4
 SHT_TRIANGLE_ONLY = 0x100
 PRECISIONS = {'native': PREC_NATIVE, 'float32': PREC_NATIVE, 'float64': PREC_NATIVE,
               'f16x3': PREC_F16X3, 'bf16x3': PREC_BF16X3}

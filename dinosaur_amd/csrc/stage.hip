@@ -265,7 +265,7 @@ template <typename T>
 struct GatherArgs {
   const T* src[kMaxSources];
   T c[kMaxSources];
-  int flags[kMaxSources];  // DINO_GATHER_IDENTITY | DINO_GATHER_SCALED
+  int flags[kMaxSources];  // DINO_GATHER_IDENTITY | _SCALED | _IMPLICIT_RANGE
   int n;
   T dt;
   int64_t off, ib, ie;
@@ -273,6 +273,8 @@ struct GatherArgs {
   int tl;
   const T* ls;  // per-degree scale of the DINO_GATHER_SCALED sources, ls[i % sl]
   int64_t sl;
+  int64_t rb, re;  // the implicit range: DINO_GATHER_IMPLICIT_RANGE sources hold
+                   // values there only (zero elsewhere, not read)
 };
 
 // U = T: idx is the element; U = vector: idx is the vector (elements W idx ..)
@@ -280,7 +282,19 @@ template <typename T, typename U>
 __device__ __forceinline__ U gather_sum(const GatherArgs<T>& a, int64_t idx) {
   constexpr int W = sizeof(U) / sizeof(T);
   auto term = [&](int k) {
-    U g = reinterpret_cast<const U*>(a.src[k])[idx];
+    const int64_t e0 = idx * W;
+    const bool rng = (a.flags[k] & DINO_GATHER_IMPLICIT_RANGE) != 0;
+    U g{};
+    if (!rng || (e0 + W > a.rb && e0 < a.re)) {
+      g = reinterpret_cast<const U*>(a.src[k])[idx];
+      if constexpr (W > 1) {
+        if (rng && !(e0 >= a.rb && e0 + W <= a.re)) {
+#pragma unroll
+          for (int e = 0; e < W; ++e)
+            if (e0 + e < a.rb || e0 + e >= a.re) g[e] = (T)0;
+        }
+      }
+    }
     if (a.flags[k] & DINO_GATHER_SCALED) {
       if constexpr (W == 1) {
         g = g * a.ls[idx % a.sl];
@@ -438,10 +452,13 @@ int launch_targets(const void* gbar, int nt, void* const* outp, const double* co
 template <typename T>
 int launch_gather(const void* const* src, const double* coef, const int* flags, int ns,
                   double dt, void* out, int64_t n_out, int64_t off, int64_t ib, int64_t ie,
-                  hipStream_t s, int64_t trl, int tl, const void* ls, int64_t sl) {
+                  hipStream_t s, int64_t trl, int tl, const void* ls, int64_t sl, int64_t rb,
+                  int64_t re) {
   GatherArgs<T> args{};
   args.ls = static_cast<const T*>(ls);
   args.sl = sl;
+  args.rb = rb;
+  args.re = re;
   args.n = ns;
   args.dt = (T)dt;
   args.off = off;
@@ -621,8 +638,13 @@ extern "C" int dino_imex_adjoint_gather(const void* const* src, const double* co
                name, dtype);
   for (int k = 0; k < n_src; ++k) {
     DINO_REQUIRE(src[k], DINO_EINVAL, "%s: NULL source %d", name, k);
-    DINO_REQUIRE((flags[k] & ~(DINO_GATHER_IDENTITY | DINO_GATHER_SCALED)) == 0, DINO_EINVAL,
-                 "%s: invalid flags %d", name, flags[k]);
+    DINO_REQUIRE((flags[k] & ~(DINO_GATHER_IDENTITY | DINO_GATHER_SCALED |
+                               DINO_GATHER_IMPLICIT_RANGE)) == 0,
+                 DINO_EINVAL, "%s: invalid flags %d", name, flags[k]);
+    if (flags[k] & DINO_GATHER_IMPLICIT_RANGE) {
+      int rc = check_range(name, n, im_begin, im_end, g_origin);
+      if (rc != DINO_OK) return rc;
+    }
     DINO_REQUIRE(!(flags[k] & DINO_GATHER_SCALED) ||
                      (degree_scale && scale_len > 0 && n % scale_len == 0),
                  DINO_EINVAL, "%s: a scaled source needs a scale whose length divides %lld",
@@ -641,9 +663,11 @@ extern "C" int dino_imex_adjoint_gather(const void* const* src, const double* co
   const int64_t trl = degrees ? field_size : 0;
   if (dtype == DINO_F64)
     return launch_gather<double>(src, coef, flags, n_src, dt, out, n_out, off, ib, ie,
-                                 as_stream(stream), trl, degrees, degree_scale, scale_len);
+                                 as_stream(stream), trl, degrees, degree_scale, scale_len,
+                                 im_begin, im_end);
   return launch_gather<float>(src, coef, flags, n_src, dt, out, n_out, off, ib, ie,
-                              as_stream(stream), trl, degrees, degree_scale, scale_len);
+                              as_stream(stream), trl, degrees, degree_scale, scale_len,
+                              im_begin, im_end);
 }
 
 extern "C" int dino_imex_stage_adjoint(const void* out_cot, int n_ex, const double* a,

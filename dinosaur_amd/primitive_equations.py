@@ -635,6 +635,12 @@ class _FlatImexOps:
                        *self._compact_views(comp))
     return comp
 
+  def terms_adjoint_into(self, gcomp, ybar):
+    """ybar[divergence..ln ps] = (implicit terms)^T gcomp; the rest of ybar
+    is not written."""
+    self.op.apply_into('dino_implicit_terms_adjoint', 0.0, self.members,
+                       *self._compact_views(gcomp), *self._views(ybar))
+
   def terms_adjoint_acc(self, gcomp, ybar):
     """ybar[divergence..ln ps] += (implicit terms)^T gcomp."""
     self.op.apply_into('dino_implicit_terms_adjoint_accumulate', 0.0, self.members,

@@ -257,10 +257,12 @@ def _fused_stage_implicit_range(ops, y0, dt, ex, im):
 
 
 def _fused_gather(ops, sources, dt, like, ranged=False, scale=None):
-  """One cotangent from its pending (source, coef, identity, scaled)
-  contributions, summed in list order in one pass (at most 8 sources:
-  _fused_imex_step takes tableaus of at most 7 stages); scaled sources are
-  multiplied by the per-degree `scale` first.  None when there are none."""
+  """One cotangent from its pending (source, coef, identity, scaled,
+  implicit_only) contributions, summed in list order in one pass (at most 8
+  sources: _fused_imex_step takes tableaus of at most 7 stages); scaled
+  sources are multiplied by the per-degree `scale` first, implicit_only
+  sources hold values on the implicit range only.  None when there are
+  none."""
   from . import _lib  # pylint: disable=g-import-not-at-top
   if not sources:
     return None
@@ -269,10 +271,11 @@ def _fused_gather(ops, sources, dt, like, ranged=False, scale=None):
   out = ops.new_compact(like) if ranged else torch.empty_like(like)
   stream = ctypes.c_void_p(torch.cuda.current_stream(like.device).cuda_stream)
   flags = (ctypes.c_int * len(sources))(*[
-      (_lib.GATHER_IDENTITY if i else 0) | (_lib.GATHER_SCALED if sc else 0)
-      for _, _, i, sc in sources])
+      (_lib.GATHER_IDENTITY if i else 0) | (_lib.GATHER_SCALED if sc else 0) |
+      (_lib.GATHER_IMPLICIT_RANGE if r else 0)
+      for _, _, i, sc, r in sources])
   rc = _lib.lib().dino_imex_adjoint_gather(
-      _ptrs([t for t, _, _, _ in sources]), _dbls([c for _, c, _, _ in sources]), flags,
+      _ptrs([t for t, _, _, _, _ in sources]), _dbls([c for _, c, _, _, _ in sources]), flags,
       len(sources), float(dt), ctypes.c_void_p(out.data_ptr()), like.numel(),
       1 if ranged else 0, ops.ib, ops.ie, ops.go, ops.field_size, ops.degrees,
       ctypes.c_void_p(scale.data_ptr() if scale is not None else 0),
@@ -397,38 +400,44 @@ class _FusedImexStep(torch.autograd.Function):
     # order): the same sums, in the same order, as writing it at the output
     # combination and accumulating at every stage.
     out_sc = sc is not None
-    pend_f = {j: [(obar, b_ex[j], False, out_sc)] for j in range(n)
+    pend_f = {j: [(obar, b_ex[j], False, out_sc, False)] for j in range(n)
               if b_ex[j] and f[j] is not None}
-    pend_g = {j: [(obar, b_im[j], False, out_sc)] for j in range(n)
+    pend_g = {j: [(obar, b_im[j], False, out_sc, False)] for j in range(n)
               if b_im[j] and ctx.has_g[j]}
-    pend_y0 = [(obar, 1.0, True, out_sc)]
+    pend_y0 = [(obar, 1.0, True, out_sc, False)]
     for i in range(n - 1, 0, -1):
       fbar = _fused_gather(ops, pend_f.pop(i, None), dt, obar, scale=sc)
       gbar = _fused_gather(ops, pend_g.pop(i, None), dt, obar, ranged=True, scale=sc)
       ybar = None
+      implicit_only = False
       if f[i] is not None and fbar is not None:
         (ybar,) = torch.autograd.grad(f[i], leaves[i], fbar)
       if ctx.has_g[i] and gbar is not None:
         if ybar is None:
-          ybar = torch.zeros_like(obar)
-        ops.terms_adjoint_acc(gbar, ybar)
+          # only the implicit terms read this stage value: its cotangent lives
+          # on the implicit range (the rest of the buffer is never read)
+          ybar = torch.empty_like(obar)
+          ops.terms_adjoint_into(gbar, ybar)
+          implicit_only = True
+        else:
+          ops.terms_adjoint_acc(gbar, ybar)
       f[i] = leaves[i] = None
       del fbar, gbar
       if ybar is None:
         continue
       ybar = ybar.contiguous()
       ops.solve_adjoint_(ybar, dt * a_im[i - 1][i])
-      pend_y0.append((ybar, 1.0, True, False))
+      pend_y0.append((ybar, 1.0, True, False, implicit_only))
       for j in range(i):
         if a_ex[i - 1][j] and f[j] is not None:
-          pend_f.setdefault(j, []).append((ybar, a_ex[i - 1][j], False, False))
+          pend_f.setdefault(j, []).append((ybar, a_ex[i - 1][j], False, False, implicit_only))
         if a_im[i - 1][j] and ctx.has_g[j]:
-          pend_g.setdefault(j, []).append((ybar, a_im[i - 1][j], False, False))
+          pend_g.setdefault(j, []).append((ybar, a_im[i - 1][j], False, False, implicit_only))
     fbar0 = _fused_gather(ops, pend_f.pop(0, None), dt, obar, scale=sc)
     gbar0 = _fused_gather(ops, pend_g.pop(0, None), dt, obar, ranged=True, scale=sc)
     if f[0] is not None and fbar0 is not None:
       (gy0,) = torch.autograd.grad(f[0], leaves[0], fbar0)
-      pend_y0.append((gy0.contiguous(), 1.0, True, False))
+      pend_y0.append((gy0.contiguous(), 1.0, True, False, False))
     del fbar0
     ybar0 = _fused_gather(ops, pend_y0, dt, obar, scale=sc)
     if ctx.has_g[0] and gbar0 is not None:

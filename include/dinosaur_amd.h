@@ -328,10 +328,13 @@ int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
  * ranged != 0 writes a ranged implicit cotangent (indexing as in
  * dino_imex_stage_ranged, nothing written outside [im_begin, im_end)).
  * field_size / degrees as for dino_imex_stage_ranged: zero where the kernel
- * skips the triangle.  1 to 8 sources; out must not overlap them;
- * degree_scale (scale_len elements, scale_len dividing n) may be NULL when
- * no source is scaled. */
-enum { DINO_GATHER_IDENTITY = 1, DINO_GATHER_SCALED = 2 };
+ * skips the triangle.  A source flagged DINO_GATHER_IMPLICIT_RANGE holds
+ * values only on [im_begin, im_end) (the cotangent of a stage value that
+ * only the implicit terms read): it counts as zero elsewhere and is not
+ * read there.  1 to 8 sources; out must not overlap them; degree_scale
+ * (scale_len elements, scale_len dividing n) may be NULL when no source is
+ * scaled. */
+enum { DINO_GATHER_IDENTITY = 1, DINO_GATHER_SCALED = 2, DINO_GATHER_IMPLICIT_RANGE = 4 };
 int dino_imex_adjoint_gather(const void* const* src, const double* coef,
                              const int* flags, int n_src, double dt,
                              void* out, int64_t n, int ranged,

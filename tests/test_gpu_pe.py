@@ -336,7 +336,8 @@ def test_adjoint_gather_matches_target_passes(dtype, gpu_device):
   first contribution followed by passes accumulating the others: bitwise
   equal for full and ranged (implicit-range) outputs, identity and scaled
   sources, sources scaled per degree (as the elementwise product would
-  scale them), vector and scalar paths (including the skipped parts below
+  scale them), sources that hold values on the implicit range only (zero
+  elsewhere, not read), vector and scalar paths (including the skipped parts below
   the triangle), and nothing written outside the range (T21 Real-layout
   fields, 4 levels + tracer)."""
   import ctypes
@@ -362,21 +363,27 @@ def test_adjoint_gather_matches_target_passes(dtype, gpu_device):
             for _ in range(4)]
     for ranged in (0, 1):
       size = ie - go if ranged else n
-      for ident, scaled in (([1, 1, 1, 1], [0, 0, 0, 0]), ([0, 0, 0, 0], [0, 0, 0, 0]),
-                            ([1, 0, 1, 0], [0, 0, 0, 0]), ([1, 0, 0, 1], [1, 1, 0, 0])):
+      for ident, scaled, inrange in (
+          ([1, 1, 1, 1], [0, 0, 0, 0], [0, 0, 0, 0]), ([0, 0, 0, 0], [0, 0, 0, 0], [0, 0, 0, 0]),
+          ([1, 0, 1, 0], [0, 0, 0, 0], [0, 1, 0, 0]), ([1, 0, 0, 1], [1, 1, 0, 0], [0, 0, 1, 1])):
         nan = lambda: torch.full((size + shift,), float('nan'), device=gpu_device,  # noqa: E731
                                  dtype=dtype)[shift:]
         ref, out = nan(), nan()
         for k, src in enumerate(srcs):
           kind = ((_lib.TARGET_IDENTITY if ident[k] else 0) |
                   (_lib.TARGET_RANGED if ranged else 0) | (_lib.TARGET_ACCUMULATE if k else 0))
-          # a scaled source is the elementwise product with the per-degree scale
+          # a scaled source is the elementwise product with the per-degree scale;
+          # an implicit-range source is zero outside [ib, ie)
           x = (src.view(-1, L) * lscale).reshape(-1) if scaled[k] else src
+          if inrange[k]:
+            x = torch.zeros_like(x)
+            x[ib:ie] = src[ib:ie]
           _lib.check(lib.dino_imex_stage_adjoint_targets(
               ctypes.c_void_p(x.data_ptr()), 1, P([ref]), D([coef[k]]), I([kind]), dt, n, ib,
               ie, go, fs, L, code, stream), 'targets')
-        flags = [(_lib.GATHER_IDENTITY if i else 0) | (_lib.GATHER_SCALED if c else 0)
-                 for i, c in zip(ident, scaled)]
+        flags = [(_lib.GATHER_IDENTITY if i else 0) | (_lib.GATHER_SCALED if c else 0) |
+                 (_lib.GATHER_IMPLICIT_RANGE if r else 0)
+                 for i, c, r in zip(ident, scaled, inrange)]
         _lib.check(lib.dino_imex_adjoint_gather(
             P(srcs), D(coef), I(flags), 4, dt, ctypes.c_void_p(out.data_ptr()), n, ranged, ib,
             ie, go, fs, L, ctypes.c_void_p(lscale.data_ptr()), L, code, stream), 'gather')
@@ -392,7 +399,7 @@ def test_adjoint_gather_matches_target_passes(dtype, gpu_device):
                                       L, None, 0, code, stream)
     assert rc == _lib.DINO_EINVAL
   # a scaled source without a scale, and an unknown flag
-  for flags in ([2, 0, 0, 0], [4, 0, 0, 0]):
+  for flags in ([2, 0, 0, 0], [8, 0, 0, 0]):
     rc = lib.dino_imex_adjoint_gather(P(srcs), D(coef), I(flags), 4, dt,
                                       ctypes.c_void_p(srcs[0].data_ptr()), n, 0, ib, ie, go, fs,
                                       L, None, 0, code, stream)

@@ -293,6 +293,33 @@ def _uses(tab, n):
   return f_used, g_used
 
 
+def _stage_plan(tab, need_output=True):
+  """(need_f, need_g, graph_only) per stage of one IMEX step: compute the
+  explicit / implicit tendency of stage i, and record explicit tendency i
+  for differentiation without computing its value.  With the output read,
+  every tendency the step's formula uses is computed.  Without it (a
+  recomputation), stage i is needed if its explicit tendency enters the
+  step at all (its graph is differentiated) or its implicit tendency feeds a
+  needed later stage; an implicit tendency is computed only for a needed
+  later stage (its adjoint needs no forward value), and a recorded explicit
+  tendency that no needed stage combines is graph-only."""
+  a_ex, a_im = tab.a_ex, tab.a_im
+  n = len(tab.b_ex)
+  f_used, g_used = _uses(tab, n)
+  if need_output:
+    return list(f_used), list(g_used), [False] * n
+  need_stage = [False] * n
+  need_g = [False] * n
+  for i in range(n - 1, -1, -1):
+    feeds = any(need_stage[j + 1] and a_im[j][i] for j in range(i, n - 1))
+    need_stage[i] = f_used[i] or feeds
+    need_g[i] = g_used[i] and feeds
+  need_f = [f_used[i] and need_stage[i] for i in range(n)]
+  graph_only = [need_f[i] and not any(need_stage[j + 1] and a_ex[j][i]
+                                      for j in range(i, n - 1)) for i in range(n)]
+  return need_f, need_g, graph_only
+
+
 def _fused_forward(ops, tab, dt, y0, keep, out_scale=None, need_output=True):
   """The stages of one IMEX step on flat buffers.  keep=True records each
   explicit evaluation's autograd graph (leaf input, output) for the reverse
@@ -304,24 +331,7 @@ def _fused_forward(ops, tab, dt, y0, keep, out_scale=None, need_output=True):
   a_ex, a_im, b_ex, b_im = tab.a_ex, tab.a_im, tab.b_ex, tab.b_im
   n = len(b_ex)
   f, g, leaves = [None] * n, [None] * n, [None] * n
-  f_used, g_used = _uses(tab, n)
-  # stage i is needed if its explicit tendency is recorded, or its implicit
-  # tendency feeds a needed later stage (the output does not count here)
-  need_f, need_g = list(f_used), list(g_used)
-  if not need_output:
-    need_stage = [False] * n
-    need_g = [False] * n
-    for i in range(n - 1, -1, -1):
-      need_stage[i] = f_used[i] or any(
-          need_stage[j + 1] and a_im[j][i] for j in range(i, n - 1))
-      need_g[i] = g_used[i] and any(need_stage[j + 1] and a_im[j][i] for j in range(i, n - 1))
-    need_f = [f_used[i] and need_stage[i] for i in range(n)]
-    # a recorded explicit tendency that no needed stage combines: only its
-    # graph is differentiated, so its value need not be computed
-    graph_only = [need_f[i] and not any(need_stage[j + 1] and a_ex[j][i]
-                                        for j in range(i, n - 1)) for i in range(n)]
-  else:
-    graph_only = [False] * n
+  need_f, need_g, graph_only = _stage_plan(tab, need_output)
 
   def explicit(i, y):
     if keep:

@@ -77,3 +77,26 @@ def test_checkpointed_rollout_keep_graphs_policy():
   for bad in (-1, 'all', 1.5):
     with pytest.raises(ValueError):
       ti.checkpointed_rollout(step, x0.clone().requires_grad_(True), 6, 2, keep_graphs=bad)
+
+
+def test_recompute_stage_plan():
+  """What a recomputed IMEX step computes when its output value is unread:
+  SIL3 records f0, f1, f2 (f2 graph-only: only the output combines it), keeps
+  g0 (stages 1 and 2 read it) and skips g2, g3 and the last stage; with the
+  output read every used tendency is computed.  A two-stage tableau whose
+  second stage only feeds the output keeps just stage 0."""
+  need_f, need_g, graph_only = ti._stage_plan(ti.SIL3_TABLEAU, need_output=True)
+  assert need_f == [True, True, True, False]
+  assert need_g == [True, False, True, True]
+  assert graph_only == [False] * 4
+  need_f, need_g, graph_only = ti._stage_plan(ti.SIL3_TABLEAU, need_output=False)
+  assert need_f == [True, True, True, False]
+  assert need_g == [True, False, False, False]
+  assert graph_only == [False, False, True, False]
+  # forward-backward Euler: y1 = y0 + dt f0 + dt g1 (solve), out = y0 + dt f0 + dt g1
+  tab = ti.ImExButcherTableau(a_ex=[[1.0]], a_im=[[0.0, 1.0]], b_ex=[1.0, 0.0],
+                              b_im=[0.0, 1.0])
+  assert ti._stage_plan(tab, need_output=True) == ([True, False], [False, True],
+                                                   [False, False])
+  assert ti._stage_plan(tab, need_output=False) == ([True, False], [False, False],
+                                                    [True, False])

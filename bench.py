@@ -22,10 +22,15 @@ Legs, all in one run:
              dense (2M-1, J, L) Legendre and (I, 2M-1) Fourier matmuls
              through torch and the nodal terms as separate tensor ops,
              otherwise the same equations and rollout;
-* cpu      - the oracle's FP32 dense transforms on the host cores, a bounded
-             sample extrapolated to one member-step (forward only).
+* accuracy - the chosen and plain legs against the same workload in FP64
+             (2 members, as many steps as the timed rollout), state and
+             gradients, per field;
+* cpu      - the oracle's whole forward path (NumPy, FP32 state arrays) on
+             the host cores: two member-steps of the workload, and the T21
+             L12 day in full.
 The roofline entry times the dominant kernel live with HIP events on the
-stream the kernel runs on.
+stream the kernel runs on.  Every leg after the headline is guarded: a
+failure is recorded under config.errors and the JSON line is still printed.
 """
 from __future__ import annotations
 
@@ -365,39 +370,99 @@ def traffic_from_profiles(kernel, workload):
   return 1024.0 * (e['fetch_kib'] * e['fetch_scale'] + e['write_kib'] * e['write_scale'])
 
 
-def cpu_baseline(sample_fields=37):
-  """Oracle FP32 dense transforms on the host cores, extrapolated to one
-  member-step of the workload (forward only)."""
-  import torch
+def oracle_workload(name, layers, moist, dtype=np.float32):
+  """The seeded rest-atmosphere recipe built by the oracle alone (its own
+  grid, transforms and closed-form initial state): (equation, state, grid)
+  with FP32 state arrays."""
   sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-  from oracle import sht as osht
-  g = osht.Grid.TL255()
-  g._basis_as(np.float32)
-  rng = np.random.default_rng(0)
-  x = rng.standard_normal((sample_fields,) + g.modal_shape).astype(np.float32)
-  t0 = time.perf_counter()
-  y = g.to_nodal(x)
-  t1 = time.perf_counter()
-  g.to_modal(y)
-  t2 = time.perf_counter()
-  per_inv = (t1 - t0) / sample_fields
-  per_fwd = (t2 - t1) / sample_fields
-  n_inv, n_fwd = INV_FIELDS, FWD_FIELDS  # field transforms per explicit evaluation
-  per_member_step = 3 * (n_inv * per_inv + n_fwd * per_fwd)
-  cores = torch.get_num_threads()
+  from oracle import pe as ope, sht as osht, sigma as osig, states as ostates
+  from oracle import units as ounits
+  from dinosaur_amd import workloads
+  og = getattr(osht.Grid, name)()
+  vertical = osig.SigmaCoordinates.equidistant(layers)
+  coords = ope.Coords(og, vertical)
+  ps = ounits.SimUnits.from_si()
+  oro = workloads.scale_orography(og.to_nodal(workloads.orography_modal_coefficients(og)))
+  st, oro_nodal, tref = ostates.isothermal_rest_atmosphere(coords, ps,
+                                                           surface_height_nondim=oro)
+  modal_oro = ope.truncated_modal_orography(oro_nodal, coords)
+  dz, dd, dt_ = workloads.perturbation_arrays(og, layers, seed=0)
+  tracers, kw = {}, {}
+  if moist:
+    q = og.to_modal(workloads.humidity_nodal(og, vertical.centers, seed=200))
+    tracers = {'specific_humidity': q,
+               'specific_cloud_liquid_water_content': np.zeros_like(q),
+               'specific_cloud_ice_water_content': np.zeros_like(q)}
+    kw = dict(humidity_key='specific_humidity',
+              cloud_keys=('specific_cloud_liquid_water_content',
+                          'specific_cloud_ice_water_content'))
+  cast = lambda a: np.ascontiguousarray(a, dtype)  # noqa: E731
+  state = ope.State(cast(st.vorticity + dz), cast(st.divergence + dd),
+                    cast(st.temperature_variation + dt_), cast(st.log_surface_pressure),
+                    tracers={k: cast(v) for k, v in tracers.items()})
+  eq = ope.PrimitiveEquationsSigma(cast(tref), cast(modal_oro), coords, ps, **kw)
+  return eq, state, og
+
+
+def cpu_baseline(torch, member_steps=2):
+  """The oracle's whole forward path (explicit terms with the reference's
+  dense transforms, implicit terms and inverse, SIL3, step filter) on the
+  host cores with FP32 state arrays: the T21 L12 dry day (72 steps of 20 min)
+  in full, and `member_steps` steps of one TL255 L37 moist member (q + 2
+  clouds, dt = 5 min), the headline's member-step, forward only."""
+  sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+  from oracle import time_integration as oti
+  from dinosaur_amd import units as units_lib
+
+  def timed(name, layers, moist, dt_s, steps):
+    eq, state, og = oracle_workload(name, layers, moist)
+    dt = units_lib.SimUnits.from_si().nondimensionalize(dt_s * units_lib.scales.units.s)
+    step = oti.step_with_filters(oti.imex_rk_sil3(eq, dt), [oti.exponential_step_filter(og, dt)])
+    og._basis_as(np.float32)  # the cached transform tables are built outside the timing
+    og._basis_as(np.float64)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+      state = step(state)
+    secs = time.perf_counter() - t0
+    finite = all(np.isfinite(a).all() for a in state.arrays().values())
+    return secs, finite
+
+  c0_secs, c0_ok = timed('T21', 12, False, 1200.0, 72)
+  print(f'[bench] cpu oracle T21 L12 day: {c0_secs:.2f} s', file=sys.stderr, flush=True)
+  c4_secs, c4_ok = timed(GRID, LAYERS, True, DT_SECONDS, member_steps)
+  per_member_step = c4_secs / member_steps
+  print(f'[bench] cpu oracle {GRID} L{LAYERS} moist: {per_member_step:.2f} s per member-step',
+        file=sys.stderr, flush=True)
+  threads = torch.get_num_threads()
   try:
-    cores = len(os.sched_getaffinity(0))
+    affinity = len(os.sched_getaffinity(0))
   except AttributeError:
+    affinity = None
+  cpu_model = platform.processor() or platform.machine()
+  try:
+    with open('/proc/cpuinfo') as fh:
+      for line in fh:
+        if line.startswith('model name'):
+          cpu_model = line.split(':', 1)[1].strip()
+          break
+  except OSError:
     pass
+  blas_threads = os.environ.get('OMP_NUM_THREADS')
   return {
       'value': (DT_SECONDS / 86400.0) / per_member_step,
       'unit': 'member-sim-days/s',
-      'cores': cores,
+      'cores': int(blas_threads) if blas_threads else affinity,
       'kind': 'port',
-      'sample': (f'oracle FP32 dense-einsum TL255 transforms of {sample_fields} fields each '
-                 f'way ({t2 - t0:.1f} s), extrapolated to 3 x ({n_inv} inverse + {n_fwd} '
-                 'forward) field transforms per member-step; forward only, nodal terms '
-                 f'excluded; host {platform.processor() or platform.machine()}'),
+      'sample': (f'oracle (NumPy restatement of the reference, dense-einsum transforms as '
+                 f'stacked BLAS GEMMs, FP32 state arrays) SIL3 + step filter: {member_steps} '
+                 f'steps of one {GRID} L{LAYERS} moist member (q + 2 clouds, dt 5 min) in '
+                 f'{c4_secs:.1f} s, forward only (no adjoint); host {cpu_model}, '
+                 f'{affinity} cores visible, BLAS threads {blas_threads or "default"}'),
+      'finite': bool(c4_ok and c0_ok),
+      'config0_t21_l12_day': {'seconds': c0_secs,
+                              'sim_days_per_s': 1.0 / c0_secs,
+                              'steps': 72},
+      'torch_threads': threads,
   }
 
 
@@ -508,81 +573,21 @@ def main():
         f'{value:.3f} member-sim-days/s, graphs kept for {rollout.get("kept")} of '
         f'{rollout.get("segments")} steps', file=sys.stderr, flush=True)
   torch.cuda.empty_cache()
-  other = None
-  if not args.forward_only:
-    other_keep = 0 if rollout.get('kept') else 'auto'
-    orep = {}
-    owall, _ = time_leg(torch, step, state, coords, targets, args.steps, 1, False, other_keep,
-                        orep, warm_len=args.steps)
-    other = {'keep_graphs': other_keep, 'ms_per_step': owall / args.steps * 1e3,
-             'graphs_kept': orep.get('kept'),
-             'member_sim_days_per_s': members * args.steps * days / owall}
-    print(f'[bench] chosen, keep_graphs={other_keep!r}: {other["ms_per_step"]:.1f} ms/step, '
-          f'graphs kept for {orep.get("kept")} of {orep.get("segments")} steps',
-          file=sys.stderr, flush=True)
-    torch.cuda.empty_cache()
 
-  # dominant kernel, timed live on its launch stream
-  pcode = _lib.PRECISIONS[precision]
-  fwd_batch = members * FWD_FIELDS
-  kern = {}
-  for stage in STAGES:
-    secs, work, kind = measure_kernel(torch, coords.horizontal, fwd_batch if stage.endswith('forward')
-                                      else members * INV_FIELDS, stage, pcode)
-    kern[stage] = (secs, work, kind)
-    print(f'[bench] {stage}: {secs * 1e3:.3f} ms/launch', file=sys.stderr, flush=True)
-  dom = max(kern, key=lambda k: kern[k][0])
-  secs, work, kind = kern[dom]
-  roof = {'bound': 'hbm', 'achieved': work / secs / 1e9, 'peak': 8000.0, 'unit': 'GB/s'}
-  roof['frac'] = roof['achieved'] / roof['peak']
-  batch = members * (FWD_FIELDS if dom.endswith('forward') else INV_FIELDS)
-  roof['traffic'] = traffic_from_profiles(dom, f'{GRID} batch={batch} {precision}')
-  roof['kernel'] = dom
-  roof['algorithmic_bytes'] = work
+  # Every leg after the headline is guarded: a failure is recorded in the
+  # JSON line (config.errors) and the line is printed regardless.
+  errors = {}
 
-  # forward-only rollout replayed from a HIP graph (inference path, same state)
-  from dinosaur_amd import time_integration as ti_g
-  from dinosaur_amd import primitive_equations as pe_g
-  graphed = ti_g.GraphedStep(step, pe_g.to_flat(state))
-  y = pe_g.to_flat(state)
-  _sync(torch)
-  g0 = time.perf_counter()
-  for _ in range(4):
-    y = graphed(y)
-  _sync(torch)
-  graphed_ms = (time.perf_counter() - g0) / 4 * 1e3
-  print(f'[bench] graphed forward step: {graphed_ms:.1f} ms', file=sys.stderr, flush=True)
-  del graphed, y
-  plain = None
-  del eq, state, step, targets
-  torch.cuda.empty_cache()
-  if not args.no_plain:
-    print('[bench] plain FP32 leg (dense reference formulation)', file=sys.stderr, flush=True)
-    _, pstate, pcoords, pstep, ptargets = build(torch, 'float32', members,
-                                                impl=DenseFP32Harmonics, fused_nodal=False)
-    prollout = {}
-    pwall, _ = time_leg(torch, pstep, pstate, pcoords, ptargets, args.plain_steps, 1,
-                        args.forward_only, keep, prollout)
-    pval = members * args.plain_steps * days / pwall
-    plain = {'value': pval, 'ms_per_step': pwall / args.plain_steps * 1e3,
-             'steps': args.plain_steps, 'graphs_kept': prollout.get('kept')}
-    print(f'[bench] plain: {pwall:.3f} s, {pval:.3f} member-sim-days/s', file=sys.stderr,
-          flush=True)
-    del pstate, pstep, ptargets
-    torch.cuda.empty_cache()
-  acc = None
-  if not args.no_accuracy:
-    print('[bench] accuracy vs FP64 (2 members, 1 step)', file=sys.stderr, flush=True)
-    acc = accuracy(torch, precision)
-    print(f'[bench] accuracy: {acc}', file=sys.stderr, flush=True)
-    torch.cuda.empty_cache()
-    if not args.no_plain:
-      pacc = accuracy(torch, 'float32', impl=DenseFP32Harmonics, fused_nodal=False)
-      acc['plain_fp32'] = {k: pacc[k] for k in ('state_max_rel', 'state_rms_rel',
-                                                 'grad_max_rel', 'grad_rms_rel')}
-      print(f'[bench] plain accuracy: {acc["plain_fp32"]}', file=sys.stderr, flush=True)
+  def leg(name, fn):
+    try:
+      return fn()
+    except Exception as e:  # pylint: disable=broad-except
+      errors[name] = f'{type(e).__name__}: {e}'[:400]
+      print(f'[bench] {name} failed: {errors[name]}', file=sys.stderr, flush=True)
+      return None
+    finally:
       torch.cuda.empty_cache()
-  cpu = None if args.no_cpu else cpu_baseline()
+
   out = {
       'metric': 'TL255 L37 sim-days/s fwd+adjoint, chosen vs plain FP32, <=1e-5 rel err vs FP64',
       'value': value,
@@ -604,19 +609,113 @@ def main():
                  'rollout': {'steps': args.steps, 'keep_graphs': keep,
                              'graphs_kept': rollout.get('kept'),
                              'recomputed': (rollout.get('segments') or 0) -
-                                           (rollout.get('kept') or 0),
-                             'other_policy': other},
+                                           (rollout.get('kept') or 0)},
                  'members': members, 'grid': GRID, 'levels': LAYERS,
                  'ensemble_sim_days_per_s': args.steps * days / wall,
-                 'plain_fp32': plain,
-                 'chosen_over_plain': (value / plain['value']) if plain else None,
-                 'forward_only': args.forward_only,
-                 'graphed_forward_ms_per_step': graphed_ms, 'autotune': tune_summary,
-                 'accuracy_vs_fp64': acc},
-      'roofline': roof,
-      'cpu_baseline': cpu,
+                 'forward_only': args.forward_only, 'autotune': tune_summary,
+                 'errors': errors},
+      'roofline': None,
+      'cpu_baseline': None,
   }
-  print(json.dumps(out))
+  cfg = out['config']
+  try:
+    if not args.forward_only:
+      def other_leg():
+        other_keep = 0 if rollout.get('kept') else 'auto'
+        orep = {}
+        owall, _ = time_leg(torch, step, state, coords, targets, args.steps, 1, False,
+                            other_keep, orep, warm_len=args.steps)
+        res = {'keep_graphs': other_keep, 'ms_per_step': owall / args.steps * 1e3,
+               'graphs_kept': orep.get('kept'),
+               'member_sim_days_per_s': members * args.steps * days / owall}
+        print(f'[bench] chosen, keep_graphs={other_keep!r}: {res["ms_per_step"]:.1f} ms/step, '
+              f'graphs kept for {orep.get("kept")} of {orep.get("segments")} steps',
+              file=sys.stderr, flush=True)
+        return res
+      cfg['rollout']['other_policy'] = leg('other_policy', other_leg)
+
+    out['roofline'] = leg('roofline', lambda: dominant_kernel_roofline(
+        torch, coords.horizontal, members, precision))
+    cfg['graphed_forward_ms_per_step'] = leg('graphed_forward', lambda: graphed_forward_ms(
+        torch, step, state))
+    del eq, state, step, targets
+    torch.cuda.empty_cache()
+    plain = None
+    if not args.no_plain:
+      plain = leg('plain', lambda: plain_leg(torch, members, args.plain_steps,
+                                             args.forward_only, keep, days))
+    cfg['plain_fp32'] = plain
+    cfg['chosen_over_plain'] = (value / plain['value']) if plain else None
+    if not args.no_accuracy:
+      steps_acc = args.steps
+      print(f'[bench] accuracy vs FP64 (2 members, {steps_acc} steps)', file=sys.stderr,
+            flush=True)
+      acc = leg('accuracy', lambda: accuracy(torch, precision, steps=steps_acc))
+      print(f'[bench] accuracy: {acc}', file=sys.stderr, flush=True)
+      if acc is not None and not args.no_plain:
+        pacc = leg('accuracy_plain', lambda: accuracy(
+            torch, 'float32', steps=steps_acc, impl=DenseFP32Harmonics, fused_nodal=False))
+        if pacc is not None:
+          acc['plain_fp32'] = {k: pacc[k] for k in ('state_max_rel', 'state_rms_rel',
+                                                     'grad_max_rel', 'grad_rms_rel')}
+          print(f'[bench] plain accuracy: {acc["plain_fp32"]}', file=sys.stderr, flush=True)
+      cfg['accuracy_vs_fp64'] = acc
+    if not args.no_cpu:
+      out['cpu_baseline'] = leg('cpu_baseline', lambda: cpu_baseline(torch))
+  finally:
+    print(json.dumps(out), flush=True)
+
+
+def dominant_kernel_roofline(torch, grid, members, precision):
+  """The longest of the four transform-stage kernels at the workload's batch,
+  timed live on its launch stream with HIP events."""
+  from dinosaur_amd import _lib
+  pcode = _lib.PRECISIONS[precision]
+  kern = {}
+  for stage in STAGES:
+    batch = members * (FWD_FIELDS if stage.endswith('forward') else INV_FIELDS)
+    kern[stage] = measure_kernel(torch, grid, batch, stage, pcode)
+    print(f'[bench] {stage}: {kern[stage][0] * 1e3:.3f} ms/launch', file=sys.stderr, flush=True)
+  dom = max(kern, key=lambda k: kern[k][0])
+  secs, work, _ = kern[dom]
+  roof = {'bound': 'hbm', 'achieved': work / secs / 1e9, 'peak': 8000.0, 'unit': 'GB/s'}
+  roof['frac'] = roof['achieved'] / roof['peak']
+  batch = members * (FWD_FIELDS if dom.endswith('forward') else INV_FIELDS)
+  roof['traffic'] = traffic_from_profiles(dom, f'{GRID} batch={batch} {precision}')
+  roof['kernel'] = dom
+  roof['algorithmic_bytes'] = work
+  roof['ms_per_launch'] = secs * 1e3
+  return roof
+
+
+def graphed_forward_ms(torch, step, state):
+  """Forward-only steps replayed from a HIP graph (inference path)."""
+  from dinosaur_amd import primitive_equations as pe_g
+  from dinosaur_amd import time_integration as ti_g
+  graphed = ti_g.GraphedStep(step, pe_g.to_flat(state))
+  y = pe_g.to_flat(state)
+  _sync(torch)
+  g0 = time.perf_counter()
+  for _ in range(4):
+    y = graphed(y)
+  _sync(torch)
+  ms = (time.perf_counter() - g0) / 4 * 1e3
+  print(f'[bench] graphed forward step: {ms:.1f} ms', file=sys.stderr, flush=True)
+  return ms
+
+
+def plain_leg(torch, members, steps, forward_only, keep, days):
+  print('[bench] plain FP32 leg (dense reference formulation)', file=sys.stderr, flush=True)
+  _, pstate, pcoords, pstep, ptargets = build(torch, 'float32', members,
+                                              impl=DenseFP32Harmonics, fused_nodal=False)
+  prollout = {}
+  pwall, _ = time_leg(torch, pstep, pstate, pcoords, ptargets, steps, 1, forward_only, keep,
+                      prollout)
+  pval = members * steps * days / pwall
+  print(f'[bench] plain: {pwall:.3f} s, {pval:.3f} member-sim-days/s', file=sys.stderr,
+        flush=True)
+  return {'value': pval, 'ms_per_step': pwall / steps * 1e3, 'steps': steps,
+          'graphs_kept': prollout.get('kept')}
 
 
 if __name__ == '__main__':

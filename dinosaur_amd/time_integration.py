@@ -730,11 +730,10 @@ def checkpointed_rollout(step_fn, x, steps: int, checkpoint_every: int, keep_gra
 
   keep_graphs: segments that record their graphs in the forward pass instead
   of being recomputed -- an int (the first `keep_graphs` segments) or 'auto'
-  (every segment for which free device memory still leaves room for two more
-  segment graphs and 4 GiB, the first segment's graph size estimated as
-  12 x its input bytes per step and later ones as the largest graph kept so
-  far).  Kept segments skip the recomputation; gradients are bitwise the
-  same.  `report`, if given, receives {'segments', 'kept'}."""
+  (decided per segment by `_GraphBudget` from the device's free memory and
+  the measured high-water mark of the segments kept so far).  Kept segments
+  skip the recomputation; gradients are bitwise the same.  `report`, if
+  given, receives {'segments', 'kept'}."""
   if steps % checkpoint_every:
     raise ValueError(f'inconsistent steps={steps} and checkpoint_every={checkpoint_every}')
   if not (keep_graphs == 'auto' or (isinstance(keep_graphs, int) and keep_graphs >= 0)):
@@ -766,40 +765,101 @@ def checkpointed_rollout(step_fn, x, steps: int, checkpoint_every: int, keep_gra
     return tuple(out)
 
   n_seg = steps // checkpoint_every
-  kept, graph_bytes = 0, None
+  budget = _GraphBudget(keep_graphs, leaves, checkpoint_every)
   for _ in range(n_seg):
     fn = functools.partial(segment, rebuild)
     if torch.is_grad_enabled() and any(t.requires_grad for t in leaves):
-      if _keep_segment(keep_graphs, kept, graph_bytes, leaves, checkpoint_every):
-        dev = leaves[0].device
-        before = torch.cuda.memory_allocated(dev) if dev.type == 'cuda' else 0
-        leaves = list(fn(*leaves))
-        if dev.type == 'cuda':
-          graph_bytes = max(graph_bytes or 0, torch.cuda.memory_allocated(dev) - before)
-        kept += 1
+      if budget.keep_next():
+        leaves = list(budget.run_kept(fn, leaves))
       else:
         leaves = list(_CheckpointSegment.apply(fn, *leaves))
     else:
       leaves = list(fn(*leaves))
     rebuild = structure['rebuild']
   if report is not None:
-    report.update(segments=n_seg, kept=kept)
+    report.update(segments=n_seg, kept=budget.kept)
   return rebuild(leaves)
 
 
-def _keep_segment(policy, kept, graph_bytes, leaves, steps_per_segment) -> bool:
-  """Whether the next segment records its graph in the forward pass."""
-  if policy != 'auto':
-    return kept < policy
+class _CudaMemory:
+  """Device memory queries used by `_GraphBudget` (a seam for host tests)."""
+
+  def __init__(self, dev):
+    self.dev = dev
+
+  def free(self) -> int:
+    # memory the driver can still hand out; blocks cached by torch's
+    # allocator are not counted: they are sized for earlier requests and may
+    # not hold a large new one
+    return torch.cuda.mem_get_info(self.dev)[0]
+
+  def allocated(self) -> int:
+    return torch.cuda.memory_allocated(self.dev)
+
+  def reset_peak(self) -> None:
+    torch.cuda.reset_peak_memory_stats(self.dev)
+
+  def peak(self) -> int:
+    return torch.cuda.max_memory_allocated(self.dev)
+
+
+def _memory_for(leaves):
   dev = leaves[0].device
-  if dev.type != 'cuda':
-    return False
-  need = graph_bytes
-  if need is None:
-    need = 12 * steps_per_segment * sum(t.numel() * t.element_size() for t in leaves)
-  free, _ = torch.cuda.mem_get_info(dev)
-  free += torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev)
-  return free - need >= 2 * need + (4 << 30)
+  return _CudaMemory(dev) if dev.type == 'cuda' else None
+
+
+class _GraphBudget:
+  """Which rollout segments record their graphs in the forward pass.
+
+  An int policy keeps the first `policy` segments.  'auto' keeps a segment
+  while the device's free memory covers
+
+      held + 2 * peak + 4 GiB
+
+  where `held` is what a kept segment leaves allocated (its graph) and
+  `peak` its high-water mark above the allocation it started from (the
+  graph plus the step's temporaries: transform workspaces, nodal buffers).
+  One `peak` is the segment itself; the other is reserved for the backward
+  pass, which recomputes the segments that were not kept while every kept
+  graph is still alive, and allocates cotangents of the same sizes.  Before
+  any segment has been measured both are estimated from the input bytes
+  (12x per step held, 16x peak).  Measurements take the largest seen so
+  far.  Resets the device's peak-memory statistics around kept segments."""
+
+  RESERVE = 4 << 30
+
+  def __init__(self, policy, leaves, steps_per_segment, memory=None):
+    self.policy = policy
+    self.kept = 0
+    self.mem = memory if memory is not None else (
+        _memory_for(leaves) if policy == 'auto' else None)
+    nbytes = sum(t.numel() * t.element_size() for t in leaves)
+    self.held = 12 * steps_per_segment * nbytes
+    self.peak = 16 * steps_per_segment * nbytes
+    self.measured = False
+
+  def keep_next(self) -> bool:
+    if self.policy != 'auto':
+      return self.kept < self.policy
+    if self.mem is None:
+      return False
+    return self.mem.free() >= self.held + 2 * self.peak + self.RESERVE
+
+  def run_kept(self, fn, leaves):
+    mem = self.mem
+    if mem is not None:
+      before = mem.allocated()
+      mem.reset_peak()
+    out = fn(*leaves)
+    if mem is not None:
+      held = mem.allocated() - before
+      peak = mem.peak() - before
+      if self.measured:
+        self.held, self.peak = max(self.held, held), max(self.peak, peak)
+      else:
+        self.held, self.peak, self.measured = held, peak, True
+    self.kept += 1
+    return out
 
 
 class _CheckpointSegment(torch.autograd.Function):

@@ -255,14 +255,23 @@ class Grid:
     return cache[key]
 
   # ---- transforms (spherical_harmonic.py:262-280, 1017-1031)
+  # The reference's two einsums per direction, evaluated as stacked matrix
+  # products (one BLAS GEMM per zonal wavenumber for the Legendre sums, one
+  # for the Fourier sums over the whole batch), in the same order: Legendre
+  # then Fourier for to_nodal, Fourier then Legendre for to_modal.
   def to_nodal(self, x):
     if isinstance(x, dict):
       return {k: self.to_nodal(v) for k, v in x.items()}
     if isinstance(x, (tuple, list)):
       return type(x)(self.to_nodal(v) for v in x)
     f, p, _ = self._basis_as(x.dtype)
-    px = np.einsum('mjl,...ml->...mj', p, x, optimize=True)
-    return np.einsum('im,...mj->...ij', f, px, optimize=True)
+    lead = x.shape[:-2]
+    (n_r, n_l), (n_i, n_j) = self.modal_shape, self.nodal_shape
+    xb = np.ascontiguousarray(np.moveaxis(x.reshape((-1, n_r, n_l)), 0, -1))  # (m, l, b)
+    b = xb.shape[-1]
+    px = np.matmul(p, xb)                                       # 'mjl,...ml->...mj'
+    y = (f @ px.reshape(n_r, n_j * b)).reshape(n_i, n_j, b)     # 'im,...mj->...ij'
+    return np.ascontiguousarray(np.moveaxis(y, -1, 0)).reshape(lead + (n_i, n_j))
 
   def to_modal(self, z):
     if isinstance(z, dict):
@@ -270,9 +279,14 @@ class Grid:
     if isinstance(z, (tuple, list)):
       return type(z)(self.to_modal(v) for v in z)
     f, p, w = self._basis_as(z.dtype)
-    wz = w * z
-    fwz = np.einsum('im,...ij->...mj', f, wz, optimize=True)
-    return np.einsum('mjl,...mj->...ml', p, fwz, optimize=True)
+    lead = z.shape[:-2]
+    (n_r, n_l), (n_i, n_j) = self.modal_shape, self.nodal_shape
+    wz = (w * z).reshape((-1, n_i, n_j))
+    b = wz.shape[0]
+    wzb = np.ascontiguousarray(np.moveaxis(wz, 0, -1)).reshape(n_i, n_j * b)
+    fwz = (f.T @ wzb).reshape(n_r, n_j, b)                      # 'im,...ij->...mj'
+    out = np.matmul(p.transpose(0, 2, 1), fwz)                  # 'mjl,...mj->...ml'
+    return np.ascontiguousarray(np.moveaxis(out, -1, 0)).reshape(lead + (n_r, n_l))
 
   # ---- spectral operators (spherical_harmonic.py:1033-1140)
   def laplacian(self, x):

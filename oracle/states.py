@@ -11,7 +11,6 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import units as u
 from .pe import State
 
 
@@ -31,13 +30,13 @@ def isothermal_rest_atmosphere(coords, physics_specs, tref=288.0, p0=1e5,
   grid = coords.horizontal
   lon, sin_lat = grid.nodal_mesh
   lat = np.arcsin(sin_lat)
-  tref_nd = u.kelvin(tref)
-  p0_nd = u.pascal(p0)
+  tref_nd = physics_specs.nondim(tref, temperature=1)
+  p0_nd = physics_specs.nondim(p0, length=-1, time=-2, mass=1)
   if surface_height_nondim is None:
     orography = np.zeros_like(lat)
   else:
     orography = np.asarray(surface_height_nondim, np.float64)
-  altitude_m = u.dim(orography, length=1)
+  altitude_m = physics_specs.dim(orography, length=1)
   sp = p0_nd * np.ones(coords.surface_nodal_shape) * relative_pressure(altitude_m)
   nodal_vort = np.zeros(coords.nodal_shape)
   modal_vort = grid.to_modal(nodal_vort)
@@ -54,11 +53,12 @@ def isothermal_rest_atmosphere(coords, physics_specs, tref=288.0, p0=1e5,
 def steady_state_jw(coords, physics_specs, u0=35.0, p0=1e5, t0=288.0,
                     delta_t=4.8e5, gamma=0.005, eta_tropo=0.2, eta0=0.252):
   """Returns (state, nodal_orography, reference_temperatures, geopotential)."""
-  u0 = u.nondim(u0, length=1, time=-1)
-  t0 = u.kelvin(t0)
-  delta_t = u.kelvin(delta_t)
-  p0 = u.pascal(p0)
-  gamma = u.nondim(gamma, length=-1, temperature=1)
+  nd = physics_specs.nondim
+  u0 = nd(u0, length=1, time=-1)
+  t0 = nd(t0, temperature=1)
+  delta_t = nd(delta_t, temperature=1)
+  p0 = nd(p0, length=-1, time=-2, mass=1)
+  gamma = nd(gamma, length=-1, temperature=1)
   a = physics_specs.radius
   g = physics_specs.g
   r_gas = physics_specs.R
@@ -120,7 +120,7 @@ def steady_state_jw(coords, physics_specs, u0=35.0, p0=1e5, t0=288.0,
 def baroclinic_perturbation_jw(coords, physics_specs, u_perturb=1.0,
                                lon_location=np.pi / 9, lat_location=2 * np.pi / 9,
                                perturbation_radius=0.1):
-  u_p = u.nondim(u_perturb, length=1, time=-1)
+  u_p = physics_specs.nondim(u_perturb, length=1, time=-1)
   a = physics_specs.radius
   grid = coords.horizontal
   lon, sin_lat = grid.nodal_mesh

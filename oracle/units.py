@@ -22,23 +22,34 @@ KAPPA = 2 / 7                        # scales.py:61
 R_SI = CP_SI * KAPPA                 # scales.py:70
 R_VAPOR_SI = 461.0                   # scales.py:74
 
-LENGTH_UNIT = RADIUS_M
-TIME_UNIT = 1 / 2 / OMEGA_PER_S      # scales.py:206-211
-MASS_UNIT = 1.0
-TEMPERATURE_UNIT = 1.0
+@dataclasses.dataclass(frozen=True)
+class Scale:
+  """Units of length, time, mass and temperature (scales.py:186-228)."""
+  length: float = RADIUS_M
+  time: float = 1 / 2 / OMEGA_PER_S      # scales.py:206-211
+  mass: float = 1.0
+  temperature: float = 1.0
+
+  def factor(self, length=0, time=0, mass=0, temperature=0) -> float:
+    return ((self.length ** length) * (self.time ** time) * (self.mass ** mass)
+            * (self.temperature ** temperature))
 
 
-def nondim(value: float, length=0, time=0, mass=0, temperature=0) -> float:
+DEFAULT_SCALE = Scale()
+LENGTH_UNIT = DEFAULT_SCALE.length
+TIME_UNIT = DEFAULT_SCALE.time
+MASS_UNIT = DEFAULT_SCALE.mass
+TEMPERATURE_UNIT = DEFAULT_SCALE.temperature
+
+
+def nondim(value: float, length=0, time=0, mass=0, temperature=0,
+           scale: Scale = DEFAULT_SCALE) -> float:
   """value_SI / scale**dims (scales.py:168-181)."""
-  factor = (LENGTH_UNIT ** length) * (TIME_UNIT ** time) * (
-      MASS_UNIT ** mass) * (TEMPERATURE_UNIT ** temperature)
-  return value / factor
+  return value / scale.factor(length, time, mass, temperature)
 
 
-def dim(value, length=0, time=0, mass=0, temperature=0):
-  factor = (LENGTH_UNIT ** length) * (TIME_UNIT ** time) * (
-      MASS_UNIT ** mass) * (TEMPERATURE_UNIT ** temperature)
-  return value * factor
+def dim(value, length=0, time=0, mass=0, temperature=0, scale: Scale = DEFAULT_SCALE):
+  return value * scale.factor(length, time, mass, temperature)
 
 
 def seconds(s):
@@ -93,16 +104,26 @@ class SimUnits:
   def Cp_vapor(self):
     return self.water_vapor_isobaric_heat_capacity
 
+  scale: Scale = DEFAULT_SCALE
+
+  def nondim(self, value, **dims):
+    return nondim(value, scale=self.scale, **dims)
+
+  def dim(self, value, **dims):
+    return dim(value, scale=self.scale, **dims)
+
   @classmethod
-  def from_si(cls, ideal_gas_constant_si=None, kappa_si=None):
+  def from_si(cls, ideal_gas_constant_si=None, kappa_si=None, scale: Scale = DEFAULT_SCALE):
     kappa = KAPPA if kappa_si is None else kappa_si
     r_si = (CP_SI * KAPPA) if ideal_gas_constant_si is None else ideal_gas_constant_si
+    nd = lambda v, **d: nondim(v, scale=scale, **d)  # noqa: E731
     return cls(
-        radius=nondim(RADIUS_M, length=1),
-        angular_velocity=nondim(OMEGA_PER_S, time=-1),
-        gravity_acceleration=nondim(GRAVITY_SI, length=1, time=-2),
-        ideal_gas_constant=nondim(r_si, length=2, time=-2, temperature=-1),
-        water_vapor_gas_constant=nondim(R_VAPOR_SI, length=2, time=-2, temperature=-1),
-        water_vapor_isobaric_heat_capacity=nondim(CP_VAPOR_SI, length=2, time=-2, temperature=-1),
+        radius=nd(RADIUS_M, length=1),
+        angular_velocity=nd(OMEGA_PER_S, time=-1),
+        gravity_acceleration=nd(GRAVITY_SI, length=1, time=-2),
+        ideal_gas_constant=nd(r_si, length=2, time=-2, temperature=-1),
+        water_vapor_gas_constant=nd(R_VAPOR_SI, length=2, time=-2, temperature=-1),
+        water_vapor_isobaric_heat_capacity=nd(CP_VAPOR_SI, length=2, time=-2, temperature=-1),
         kappa=kappa,
+        scale=scale,
     )

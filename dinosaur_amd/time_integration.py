@@ -621,31 +621,147 @@ def step_with_filters(step_fn, filters):
   return _step_fn
 
 
-def repeated(fn, steps: int, scan_fn=None):
+def _is_state(x):
+  return hasattr(x, '_ARRAYS') and hasattr(x, 'tracers')
+
+
+def _tree_map_leaves(fn, *trees):
+  """fn over matching leaves of pytrees made of tensors, arrays, numbers,
+  None, States (array leaves and tracers; sim_time is treated as a leaf),
+  tuples, lists and dicts."""
+  first = trees[0]
+  if first is None:
+    return None
+  if _is_state(first):
+    kw = {f: fn(*[getattr(t, f) for t in trees]) for f in first._ARRAYS}
+    kw['tracers'] = {k: fn(*[t.tracers[k] for t in trees]) for k in first.tracers}
+    kw['sim_time'] = (None if first.sim_time is None
+                      else fn(*[t.sim_time for t in trees]))
+    return type(first)(**kw)
+  if isinstance(first, (tuple, list)):
+    return type(first)(_tree_map_leaves(fn, *xs) for xs in zip(*trees))
+  if isinstance(first, dict):
+    return {k: _tree_map_leaves(fn, *[t[k] for t in trees]) for k in first}
+  return fn(*trees)
+
+
+def _as_stackable(x):
+  if isinstance(x, torch.Tensor):
+    return x
+  return torch.as_tensor(x)
+
+
+def _stack_outputs(ys):
+  """Per-step outputs -> one pytree with a leading step axis (lax.scan's ys)."""
+  if not ys:
+    return None
+  return _tree_map_leaves(lambda *leaves: torch.stack([_as_stackable(v) for v in leaves]), *ys)
+
+
+def _concat_outputs(chunks):
+  if not chunks or chunks[0] is None:
+    return None
+  return _tree_map_leaves(lambda *leaves: torch.cat(leaves), *chunks)
+
+
+def _index_xs(xs, i):
+  return None if xs is None else _tree_map_leaves(lambda a: a[i], xs)
+
+
+def scan(f, init, xs=None, length=None):
+  """lax.scan semantics on the host loop: f(carry, x) -> (carry, y); returns
+  (carry, ys) with ys stacked along a new leading axis (None if f returns
+  None)."""
+  if length is None:
+    if xs is None:
+      raise ValueError('scan needs `length` when xs is None')
+    leaves = []
+    _tree_map_leaves(lambda a: leaves.append(a), xs)
+    length = len(leaves[0])
+  carry, ys = init, []
+  for i in range(length):
+    carry, y = f(carry, _index_xs(xs, i))
+    ys.append(y)
+  if all(y is None for y in ys):
+    return carry, None
+  return carry, _stack_outputs(ys)
+
+
+def repeated(fn, steps: int, scan_fn=scan):
+  """fn applied `steps` times (time_integration.py:508-520)."""
   if steps == 1:
     return fn
 
   def f_repeated(x):
-    for _ in range(steps):
-      x = fn(x)
-    return x
+    x_final, _ = scan_fn(lambda c, _: (fn(c), None), x, None, steps)
+    return x_final
   return f_repeated
 
 
 def trajectory_from_step(step_fn, outer_steps: int, inner_steps: int, *,
                          start_with_input: bool = False, post_process_fn=lambda x: x,
-                         outer_scan_fn=None, inner_scan_fn=None):
-  """Returns x -> (final state, list of outer frames) (time_integration.py:523-564)."""
-  inner = repeated(step_fn, inner_steps)
+                         outer_scan_fn=scan, inner_scan_fn=scan):
+  """x -> (final state, trajectory): `outer_steps` frames, `inner_steps`
+  steps apart, stacked along a leading axis (time_integration.py:523-564).
+  Frames are the states after each outer step, or before it with
+  start_with_input, passed through post_process_fn."""
+  if inner_steps != 1:
+    step_fn = repeated(step_fn, inner_steps, inner_scan_fn)
+
+  def step(carry_in, _):
+    carry_out = step_fn(carry_in)
+    frame = carry_in if start_with_input else carry_out
+    return carry_out, post_process_fn(frame)
 
   def multistep(x):
-    frames = []
-    for _ in range(outer_steps):
-      y = inner(x)
-      frames.append(post_process_fn(x if start_with_input else y))
-      x = y
-    return x, frames
+    return outer_scan_fn(step, x, None, outer_steps)
   return multistep
+
+
+def _checkpoint(fn):
+  """Non-reentrant activation checkpointing of fn (torch.utils.checkpoint):
+  the forward keeps no intermediate values; the backward pass recomputes
+  them.  Outputs may be any pytree."""
+
+  def wrapped(*args):
+    if torch.is_grad_enabled():
+      return torch.utils.checkpoint.checkpoint(fn, *args, use_reentrant=False)
+    return fn(*args)
+  return wrapped
+
+
+def nested_checkpoint_scan(f, init, xs=None, length=None, *, nested_lengths,
+                           scan_fn=scan, checkpoint_fn=_checkpoint):
+  """scan with recursive checkpointing (time_integration.py:573-637): the
+  same (carry, stacked outputs) as scan(f, init, xs, length), computed as
+  nested scans whose inner scans are checkpointed, so gradients keep
+  O(max(nested_lengths)) intermediate states and recompute each level's
+  forward pass once.  scan_fn and checkpoint_fn replace the scan and the
+  checkpoint transform (defaults: `scan` and non-reentrant
+  torch.utils.checkpoint)."""
+  total = math.prod(nested_lengths)
+  if length is not None and length != total:
+    raise ValueError(f'inconsistent {length=} and {nested_lengths=}')
+
+  def nested_reshape(a):
+    a = torch.as_tensor(a)
+    return a.reshape(tuple(nested_lengths) + tuple(a.shape[1:]))
+  sub_xs = None if xs is None else _tree_map_leaves(nested_reshape, xs)
+  return _inner_nested_scan(f, init, sub_xs, list(nested_lengths), scan_fn, checkpoint_fn)
+
+
+def _inner_nested_scan(f, init, xs, lengths, scan_fn, checkpoint_fn):
+  if len(lengths) == 1:
+    return scan_fn(f, init, xs, lengths[0])
+
+  @checkpoint_fn
+  def sub_scans(carry, sub_xs):
+    return _inner_nested_scan(f, carry, sub_xs, lengths[1:], scan_fn, checkpoint_fn)
+
+  carry, out = scan_fn(sub_scans, init, xs, lengths[0])
+  if out is None:
+    return carry, None
+  return carry, _tree_map_leaves(lambda a: a.reshape((-1,) + tuple(a.shape[2:])), out)
 
 
 class GraphedStep:
@@ -893,46 +1009,6 @@ class _CheckpointSegment(torch.autograd.Function):
                                 allow_unused=True) if pairs and want else [None] * len(want)
     it = iter(grads)
     return (None,) + tuple(next(it) if t.requires_grad else None for t in inputs)
-
-
-def nested_checkpoint_scan(f, init, xs, length=None, *, nested_lengths, scan_fn=None,
-                           checkpoint_fn=None):
-  """scan with recursive checkpointing (time_integration.py:573-637).
-
-  ``f(carry, x) -> (carry, y)``; xs is None or a sequence of length
-  prod(nested_lengths).  Returns (carry, list of outputs)."""
-  total = math.prod(nested_lengths)
-  if length is not None and length != total:
-    raise ValueError(f'inconsistent {length=} and {nested_lengths=}')
-  xs_list = [None] * total if xs is None else list(xs)
-
-  def run(carry, lo, lengths):
-    if len(lengths) == 1:
-      outs = []
-      for i in range(lengths[0]):
-        carry, y = f(carry, xs_list[lo + i])
-        outs.append(y)
-      return carry, outs
-    sub = math.prod(lengths[1:])
-    outs = []
-    for i in range(lengths[0]):
-      start = lo + i * sub
-      leaves, rebuild = _flatten_state(carry)
-
-      def seg(*ls, start=start, rebuild=rebuild):
-        c, o = run(rebuild(list(ls)), start, lengths[1:])
-        cl, _ = _flatten_state(c)
-        seg.outs = o
-        return tuple(cl)
-      if torch.is_grad_enabled() and any(t.requires_grad for t in leaves):
-        new = torch.utils.checkpoint.checkpoint(seg, *leaves, use_reentrant=False)
-      else:
-        new = seg(*leaves)
-      carry = rebuild(list(new))
-      outs.extend(seg.outs)
-    return carry, outs
-
-  return run(init, 0, list(nested_lengths))
 
 
 # ---------------------------------------------------------------- digital filter initialization

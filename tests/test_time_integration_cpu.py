@@ -233,3 +233,151 @@ def test_checkpointed_rollout_auto_policy_with_device_memory(monkeypatch):
   # keep while free >= 10 + 2 * 15 + 4 = 44 GiB: free 100, 90, ..., 50 -> 6 kept
   assert rep == {'segments': 8, 'kept': 6}
   assert torch.equal(y.detach(), y_ref.detach()) and torch.equal(g, g_ref)
+
+
+# ---------------------------------------------------------------- scan drivers
+
+
+def _plain_scan(f, init, xs, length):
+  carry, ys = init, []
+  for i in range(length):
+    carry, y = f(carry, None if xs is None else xs[i])
+    ys.append(y)
+  return carry, ys
+
+
+@pytest.mark.parametrize('f,init,xs,lengths', [
+    (lambda x, _: (x + 1, x ** 2), 1, None, [8]),
+    (lambda x, _: (x + 1, x ** 2), 1, None, [2, 4]),
+    (lambda x, _: (x + 1, x ** 2), 1, None, [1, 2, 2, 2]),
+    (lambda x, y: (2 * x, x + y), 1, np.arange(8), [8]),
+    (lambda x, y: (2 * x, x + y), 1, np.arange(8), [2, 4]),
+    (lambda x, y: (2 * x, x + y), 1, np.arange(8), [2, 2, 2, 1]),
+])
+def test_nested_checkpoint_scan_matches_scan(f, init, xs, lengths):
+  """time_integration_test.py:278-300: same carry and stacked outputs as an
+  unnested scan."""
+  length = int(np.prod(lengths))
+  exp_c, exp_ys = _plain_scan(f, init, xs, length)
+  c, ys = ti.nested_checkpoint_scan(f, init, xs, length, nested_lengths=lengths)
+  np.testing.assert_allclose(np.asarray(c), np.asarray(exp_c))
+  np.testing.assert_allclose(ys.numpy(), np.asarray([np.asarray(y) for y in exp_ys]))
+  with pytest.raises(ValueError):
+    ti.nested_checkpoint_scan(f, init, xs, length + 1, nested_lengths=lengths)
+
+
+def test_nested_checkpoint_scan_gradients_and_recomputation():
+  """Gradients of a loss on the final carry and on every per-step output,
+  with respect to the initial carry, the scanned inputs and a parameter,
+  equal those of plain autograd through an unnested loop; the backward pass
+  recomputes the checkpointed inner scans (f runs again)."""
+  torch.manual_seed(1)
+  w = (torch.randn(8, 8, dtype=torch.float64) * 0.3).requires_grad_(True)
+  calls = [0]
+
+  def f(c, x):
+    calls[0] += 1
+    c = torch.tanh(c @ w) + 0.1 * c + x
+    return c, (c ** 2).sum(-1)
+
+  c0 = torch.randn(4, 8, dtype=torch.float64)
+  xs = torch.randn(12, 4, 8, dtype=torch.float64) * 0.1
+  ywt = torch.randn(12, 4, dtype=torch.float64)
+
+  def loss_and_grads(runner):
+    ci = c0.clone().requires_grad_(True)
+    xi = xs.clone().requires_grad_(True)
+    c, ys = runner(ci, xi)
+    loss = (c ** 2).sum() + (ys * ywt).sum()
+    return (c.detach(), ys.detach()) + torch.autograd.grad(loss, (ci, xi, w))
+
+  def plain(ci, xi):
+    c, ys = _plain_scan(f, ci, xi, 12)
+    return c, torch.stack(ys)
+
+  ref = loss_and_grads(plain)
+  for lengths in ([3, 4], [2, 3, 2], [12]):
+    calls[0] = 0
+    got = loss_and_grads(lambda ci, xi: ti.nested_checkpoint_scan(
+        f, ci, xi, 12, nested_lengths=lengths))
+    for a, b in zip(got, ref):
+      torch.testing.assert_close(a, b, rtol=0, atol=1e-14)
+    # each level of checkpointing re-runs (at most) the forward pass once
+    # more; torch's recomputation stops early once it has what it needs
+    if len(lengths) == 1:
+      assert calls[0] == 12
+    else:
+      assert 12 < calls[0] <= 12 * len(lengths), (lengths, calls[0])
+
+
+def test_nested_checkpoint_scan_custom_scan_and_checkpoint_functions():
+  """scan_fn and checkpoint_fn are used as given (not ignored)."""
+  seen = {'scan': 0, 'checkpoint': 0}
+
+  def my_scan(f, init, xs, length):
+    seen['scan'] += 1
+    return ti.scan(f, init, xs, length)
+
+  def my_checkpoint(fn):
+    seen['checkpoint'] += 1
+    return fn
+
+  c, ys = ti.nested_checkpoint_scan(lambda x, _: (x + 1, x), 0, None, 6, nested_lengths=[2, 3],
+                                    scan_fn=my_scan, checkpoint_fn=my_checkpoint)
+  assert c == 6 and ys.tolist() == [0, 1, 2, 3, 4, 5]
+  assert seen == {'scan': 3, 'checkpoint': 1}
+
+
+def test_repeated_and_trajectory_from_step():
+  """time_integration.py:508-564: repeated applies fn `steps` times (the
+  function itself for one step); trajectories have outer_steps frames,
+  inner_steps apart, after each outer step or before it with
+  start_with_input, through post_process_fn, stacked along a leading axis."""
+  inc = lambda x: x + 1  # noqa: E731
+  assert ti.repeated(inc, 1) is inc
+  assert ti.repeated(inc, 5)(torch.tensor(0.0)).item() == 5.0
+  final, frames = ti.trajectory_from_step(inc, 3, 2)(torch.tensor(0.0))
+  assert final.item() == 6.0 and frames.tolist() == [2.0, 4.0, 6.0]
+  final, frames = ti.trajectory_from_step(inc, 3, 2, start_with_input=True)(torch.tensor(0.0))
+  assert final.item() == 6.0 and frames.tolist() == [0.0, 2.0, 4.0]
+  _, frames = ti.trajectory_from_step(inc, 4, 1, post_process_fn=lambda x: {'y': 10 * x})(
+      torch.tensor(1.0))
+  assert frames['y'].tolist() == [20.0, 30.0, 40.0, 50.0]
+
+
+def test_trajectory_of_states_stacks_leaves_and_sim_time():
+  from dinosaur_amd import primitive_equations as pe
+  st = pe.State(torch.zeros(2, 3, 4), torch.zeros(2, 3, 4), torch.zeros(2, 3, 4),
+                torch.zeros(1, 3, 4), tracers={'q': torch.zeros(2, 3, 4)}, sim_time=0.0)
+
+  def step(s):
+    return pe.State(s.vorticity + 1, s.divergence, s.temperature_variation,
+                    s.log_surface_pressure, tracers={'q': s.tracers['q'] + 2},
+                    sim_time=s.sim_time + 0.5)
+  final, traj = ti.trajectory_from_step(step, 3, 1)(st)
+  assert final.sim_time == 1.5
+  assert tuple(traj.vorticity.shape) == (3, 2, 3, 4)
+  assert traj.vorticity[:, 0, 0, 0].tolist() == [1.0, 2.0, 3.0]
+  assert traj.tracers['q'][:, 0, 0, 0].tolist() == [2.0, 4.0, 6.0]
+  assert traj.sim_time.tolist() == [0.5, 1.0, 1.5]
+
+
+def test_validate_state_shape_raises_state_shape_error():
+  """primitive_equations.py:87-120: StateShapeError names the field."""
+  from dinosaur_amd import coordinate_systems, primitive_equations as pe
+  from dinosaur_amd import sigma_coordinates, spherical_harmonic as sh
+  grid = sh.Grid.T21()
+  coords = coordinate_systems.CoordinateSystem(grid, sigma_coordinates.SigmaCoordinates.equidistant(4))
+  ms, ss = coords.modal_shape, coords.surface_modal_shape
+  good = dict(vorticity=torch.zeros(ms), divergence=torch.zeros(ms),
+              temperature_variation=torch.zeros(ms), log_surface_pressure=torch.zeros(ss))
+  pe.validate_state_shape(pe.State(**good, tracers={'q': torch.zeros(ms)}), coords)
+  # a leading member axis is allowed
+  pe.validate_state_shape(pe.State(**{k: v[None] for k, v in good.items()}), coords)
+  for field, shape in (('vorticity', (4, 43, 22)), ('divergence', (3,) + ms[1:]),
+                       ('log_surface_pressure', ms)):
+    bad = dict(good, **{field: torch.zeros(shape)})
+    with pytest.raises(pe.StateShapeError, match=field):
+      pe.validate_state_shape(pe.State(**bad), coords)
+  with pytest.raises(pe.StateShapeError, match='tracer q'):
+    pe.validate_state_shape(pe.State(**good, tracers={'q': torch.zeros(ss)}), coords)

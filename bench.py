@@ -164,7 +164,7 @@ def build(torch, precision, members, impl=None, fused_nodal=True, dtype=None):
 
 
 def rollout_fwd_adj(torch, step, state, coords, targets, steps, forward_only=False,
-                    keep_graphs=0, report=None):
+                    keep_graphs=0, report=None, graphed=False):
   from dinosaur_amd import primitive_equations as pe
   from dinosaur_amd import time_integration as ti
   grid = coords.horizontal
@@ -179,8 +179,11 @@ def rollout_fwd_adj(torch, step, state, coords, targets, steps, forward_only=Fal
       for _ in range(steps):
         y = step(y)
     return None
-  y = ti.checkpointed_rollout(step, s0, steps, checkpoint_every=1, keep_graphs=keep_graphs,
-                              report=report)
+  if graphed:
+    y = ti.graphed_checkpointed_rollout(step, s0, steps, checkpoint_every=1)
+  else:
+    y = ti.checkpointed_rollout(step, s0, steps, checkpoint_every=1, keep_graphs=keep_graphs,
+                                report=report)
   t18 = grid.to_nodal(y.temperature_variation[..., 18, :, :])
   z18 = grid.to_nodal(y.vorticity[..., 18, :, :])
   loss = ((t18 - targets[0]) ** 2).mean() + 1e12 * ((z18 - targets[1]) ** 2).mean()
@@ -245,15 +248,16 @@ def accuracy(torch, precision, members=2, steps=1, impl=None, fused_nodal=True):
 
 
 def time_leg(torch, step, state, coords, targets, steps, warmup, forward_only=False,
-             keep_graphs=0, report=None, warm_len=1):
+             keep_graphs=0, report=None, warm_len=1, graphed=False):
   # warm_len: steps per warm-up rollout (a rollout that keeps every graph
   # needs a full-length warm-up, or its first allocations land in the timing)
   for _ in range(warmup):
-    rollout_fwd_adj(torch, step, state, coords, targets, warm_len, forward_only, keep_graphs)
+    rollout_fwd_adj(torch, step, state, coords, targets, warm_len, forward_only, keep_graphs,
+                    graphed=graphed)
   _sync(torch)
   t0 = time.perf_counter()
   out = rollout_fwd_adj(torch, step, state, coords, targets, steps, forward_only, keep_graphs,
-                        report)
+                        report, graphed=graphed)
   _sync(torch)
   wall = time.perf_counter() - t0
   return wall, out
@@ -633,6 +637,18 @@ def main():
               file=sys.stderr, flush=True)
         return res
       cfg['rollout']['other_policy'] = leg('other_policy', other_leg)
+
+    if not args.forward_only:
+      def graphed_leg():
+        gwall, _ = time_leg(torch, step, state, coords, targets, args.steps, 1, graphed=True)
+        res = {'ms_per_step': gwall / args.steps * 1e3,
+               'member_sim_days_per_s': members * args.steps * days / gwall,
+               'form': 'forward step and adjoint step (forward recording + reverse sweep) '
+                       'replayed from two HIP graphs, one-step checkpoints'}
+        print(f'[bench] chosen, graph-captured fwd+adjoint: {res["ms_per_step"]:.1f} ms/step',
+              file=sys.stderr, flush=True)
+        return res
+      cfg['rollout']['graphed_adjoint'] = leg('graphed_adjoint', graphed_leg)
 
     out['roofline'] = leg('roofline', lambda: dominant_kernel_roofline(
         torch, coords.horizontal, members, precision))

@@ -815,6 +815,142 @@ class GraphedStep:
     return self._cls.from_flat(self._out, *self._layout, sim_time=t)
 
 
+class GraphedAdjointStep:
+  """The vector-Jacobian product of one step captured into a HIP graph.
+
+  Calling it with a state buffer x and the cotangent ybar of step(x)
+  returns xbar = (d step / d x)^T ybar.  The graph holds the step's forward
+  evaluation -- recorded for differentiation only: when the step is exactly
+  a fused IMEX step, the work whose values the reverse sweep never reads is
+  skipped, as in a checkpoint recomputation -- followed by its reverse sweep
+  (the hand-written adjoint kernels), on static input, cotangent and output
+  buffers.  Gradients equal those of the eager recomputation bitwise.  The
+  returned cotangent is a view of the static output: the next call
+  overwrites it.  Lazily created objects are built by the warm-up calls
+  before capture, so nothing allocates or copies from the host inside the
+  graph."""
+
+  def __init__(self, step_fn, example, warmup: int = 2):
+    flat = example.as_flat()
+    self._layout = flat._flat[1:]
+    self._cls = type(flat)
+    self._t0 = flat.sim_time
+    self._x = flat.flat_buffer().detach().clone()
+    self._ybar = torch.zeros_like(self._x)
+    skip = getattr(step_fn, '_output_from_fused_step', False)
+
+    def run():
+      x = self._x.detach().requires_grad_(True)
+      st = self._cls.from_flat(x, *self._layout, sim_time=self._t0)
+      with torch.enable_grad():
+        _recompute.output_unread = skip
+        try:
+          y = step_fn(st)
+        finally:
+          _recompute.output_unread = False
+        yb = y.as_flat().flat_buffer()
+        (xbar,) = torch.autograd.grad(yb, x, self._ybar)
+      return xbar
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+      for _ in range(warmup):
+        run()
+    torch.cuda.current_stream().wait_stream(side)
+    self.graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(self.graph):
+      self._xbar = run()
+
+  def __call__(self, x_buf: torch.Tensor, ybar_buf: torch.Tensor) -> torch.Tensor:
+    if x_buf.shape != self._x.shape or x_buf.dtype != self._x.dtype:
+      raise ValueError(f'graphed adjoint step expects buffers of shape {tuple(self._x.shape)} '
+                       f'and dtype {self._x.dtype}, got {tuple(x_buf.shape)} {x_buf.dtype}')
+    self._x.copy_(x_buf)
+    self._ybar.copy_(ybar_buf)
+    self.graph.replay()
+    return self._xbar
+
+
+def _graphs_for(step_fn, flat):
+  """(GraphedStep, GraphedAdjointStep) of step_fn for this buffer layout,
+  captured on first use and kept on the step function."""
+  cache = getattr(step_fn, '_graph_cache', None)
+  if cache is None:
+    cache = {}
+    try:
+      step_fn._graph_cache = cache
+    except AttributeError:  # pragma: no cover  (builtins)
+      pass
+  buf = flat.flat_buffer()
+  key = (tuple(buf.shape), buf.dtype, buf.device, flat._flat[1:], flat.sim_time)
+  g = cache.get(key)
+  if g is None:
+    g = (GraphedStep(step_fn, flat), GraphedAdjointStep(step_fn, flat))
+    cache[key] = g
+  return g
+
+
+class _GraphedSegment(torch.autograd.Function):
+  """`steps` steps replayed from a forward HIP graph; the backward pass
+  replays the forward graph to rebuild the segment's inner states and then
+  the captured adjoint graph once per step, in reverse."""
+
+  @staticmethod
+  def forward(ctx, buf, graphs, steps, layout, cls, t0):
+    fwd, _ = graphs
+    ctx.graphs, ctx.steps, ctx.layout, ctx.cls, ctx.t0 = graphs, steps, layout, cls, t0
+    ctx.save_for_backward(buf)
+    y = cls.from_flat(buf.detach(), *layout, sim_time=t0)
+    for _ in range(steps):
+      y = fwd(y)
+    return y.flat_buffer().clone()
+
+  @staticmethod
+  def backward(ctx, ybar):
+    fwd, adj = ctx.graphs
+    (buf,) = ctx.saved_tensors
+    xs = [buf.detach()]
+    y = ctx.cls.from_flat(xs[0], *ctx.layout, sim_time=ctx.t0)
+    for _ in range(ctx.steps - 1):
+      y = fwd(y)
+      xs.append(y.flat_buffer().clone())
+    g = ybar.contiguous()
+    for x in reversed(xs):
+      g = adj(x, g).clone()
+    return g, None, None, None, None, None
+
+
+def graphed_checkpointed_rollout(step_fn, x, steps: int, checkpoint_every: int):
+  """checkpointed_rollout with every step replayed from HIP graphs: the
+  forward pass replays a captured forward step, and the backward pass
+  replays the captured forward steps of each segment and then, per step in
+  reverse, the captured adjoint step (GraphedAdjointStep) -- the
+  differentiated scan body of the reference's jitted rollout
+  (time_integration.py:508-520 under jax.vjp), as two HIP graphs.  The
+  state must be flat-representable and on the GPU; the rollout keeps
+  steps / checkpoint_every segment inputs.  Gradients equal those of
+  checkpointed_rollout bitwise."""
+  if steps % checkpoint_every:
+    raise ValueError(f'inconsistent steps={steps} and checkpoint_every={checkpoint_every}')
+  flat = x.as_flat()
+  buf = flat.flat_buffer()
+  layout, cls = flat._flat[1:], type(flat)
+  graphs = _graphs_for(step_fn, flat)
+  t = flat.sim_time
+  for _ in range(steps // checkpoint_every):
+    if torch.is_grad_enabled() and buf.requires_grad:
+      buf = _GraphedSegment.apply(buf, graphs, checkpoint_every, layout, cls, t)
+    else:
+      y = cls.from_flat(buf, *layout, sim_time=t)
+      for _ in range(checkpoint_every):
+        y = graphs[0](y)
+      buf = y.flat_buffer().clone()
+    if t is not None and graphs[0]._dt is not None:
+      t = t + checkpoint_every * graphs[0]._dt
+  return cls.from_flat(buf, *layout, sim_time=t)
+
+
 def _flatten_state(x):
   """State -> (list of tensors, rebuild)."""
   if hasattr(x, '_ARRAYS') and x.flat_buffer() is not None:

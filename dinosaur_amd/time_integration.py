@@ -1045,6 +1045,12 @@ class _CudaMemory:
     # not hold a large new one
     return torch.cuda.mem_get_info(self.dev)[0]
 
+  def cached(self) -> int:
+    return torch.cuda.memory_reserved(self.dev) - torch.cuda.memory_allocated(self.dev)
+
+  def release_cache(self) -> None:
+    torch.cuda.empty_cache()
+
   def allocated(self) -> int:
     return torch.cuda.memory_allocated(self.dev)
 
@@ -1076,7 +1082,10 @@ class _GraphBudget:
   graph is still alive, and allocates cotangents of the same sizes.  Before
   any segment has been measured both are estimated from the input bytes
   (12x per step held, 16x peak).  Measurements take the largest seen so
-  far.  Resets the device's peak-memory statistics around kept segments."""
+  far.  When the driver's free memory falls short while torch's allocator
+  caches unused blocks (left by earlier work in the process), the cache is
+  returned to the driver once per rollout and the test repeated.  Resets
+  the device's peak-memory statistics around kept segments."""
 
   RESERVE = 4 << 30
 
@@ -1089,13 +1098,21 @@ class _GraphBudget:
     self.held = 12 * steps_per_segment * nbytes
     self.peak = 16 * steps_per_segment * nbytes
     self.measured = False
+    self.released = False
 
   def keep_next(self) -> bool:
     if self.policy != 'auto':
       return self.kept < self.policy
     if self.mem is None:
       return False
-    return self.mem.free() >= self.held + 2 * self.peak + self.RESERVE
+    need = self.held + 2 * self.peak + self.RESERVE
+    if self.mem.free() >= need:
+      return True
+    if not self.released and self.mem.cached() > 0:
+      self.released = True
+      self.mem.release_cache()
+      return self.mem.free() >= need
+    return False
 
   def run_kept(self, fn, leaves):
     mem = self.mem

@@ -116,6 +116,13 @@ class _FakeMemory:
   def free(self):
     return self.capacity - self.alloc - self.cache
 
+  def cached(self):
+    return self.cache
+
+  def release_cache(self):
+    self.releases = getattr(self, 'releases', 0) + 1
+    self.cache = 0
+
   def allocated(self):
     return self.alloc
 
@@ -160,11 +167,30 @@ def test_graph_budget_stops_before_the_device_is_full():
 def test_graph_budget_ignores_fragmented_cache():
   """Cached but fragmented blocks are not free memory: with 39 GiB of them
   (the state the allocator was in when a policy that counted them ran out of
-  memory) fewer segments are kept and the device never overflows."""
+  memory) the budget keeps segments only while the driver's free memory
+  admits them; the cache is returned to the driver once, when it stands in
+  the way, and the device never overflows."""
   clean = _simulate(_FakeMemory(288 * GiB, 16 * GiB, 6 * GiB), 40)
   mem = _FakeMemory(288 * GiB, 16 * GiB, 6 * GiB, cache=39 * GiB)
   frag = _simulate(mem, 40)
-  assert frag.kept < clean.kept
+  assert frag.kept == clean.kept and mem.releases == 1
+  assert mem.high_water <= mem.capacity
+  assert mem.alloc + 2 * 22 * GiB <= mem.capacity
+
+
+def test_graph_budget_with_cache_that_cannot_be_released():
+  """A cache the allocator cannot return (blocks still split by live
+  tensors) leaves the driver's free memory as it was: fewer segments are
+  kept, and none beyond what that memory admits."""
+
+  class Pinned(_FakeMemory):
+    def release_cache(self):
+      self.releases = getattr(self, 'releases', 0) + 1
+
+  clean = _simulate(_FakeMemory(288 * GiB, 16 * GiB, 6 * GiB), 40)
+  mem = Pinned(288 * GiB, 16 * GiB, 6 * GiB, cache=39 * GiB)
+  frag = _simulate(mem, 40)
+  assert frag.kept < clean.kept and mem.releases == 1
   assert mem.high_water <= mem.capacity
   assert mem.alloc + mem.cache + 2 * 22 * GiB <= mem.capacity
 

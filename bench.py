@@ -69,6 +69,10 @@ def _parse():
                   help='run one stage kernel at the bench shape, then a 256 MiB device '
                        'copy (the calibration kernel for hardware byte counters), and exit')
   ap.add_argument('--reps', type=int, default=10)
+  ap.add_argument('--profile-step', action='store_true',
+                  help='build the workload, tune, run one warm-up and one timed forward + '
+                       'adjoint step (one-step checkpoint, recomputed) and exit: the process '
+                       'to put under rocprofv3 for per-kernel times and counters')
   ap.add_argument('--workload', default='tl255', choices=['tl255', 'ensemble'],
                   help='tl255: the headline fwd+adjoint line; ensemble: the mixed-truncation '
                        'ensemble (32 T85 + 24 T42 + 8 TL127, moist L24), forward ticks')
@@ -561,6 +565,16 @@ def main():
                                 'ms': {k: round(v, 3) for k, v in r['ms'].items()}}
                            for st, r in rep.items()} for b, rep in tuned.items()}
   print(f'[bench] autotune: {tune_summary}', file=sys.stderr, flush=True)
+  if args.profile_step:
+    rollout_fwd_adj(torch, step, state, coords, targets, 1)
+    _sync(torch)
+    print('[bench] profile step: warm-up done', file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    rollout_fwd_adj(torch, step, state, coords, targets, 1)
+    _sync(torch)
+    print(f'[bench] profile step: {1e3 * (time.perf_counter() - t0):.1f} ms', file=sys.stderr,
+          flush=True)
+    return
   # One-step checkpoint segments.  The headline keeps no graph (every step is
   # recomputed in the backward pass): the configured 48 h rollout (576 steps)
   # can keep the graphs of only ~12 steps in HBM, so its rate is this one to
@@ -650,6 +664,10 @@ def main():
         return res
       cfg['rollout']['graphed_adjoint'] = leg('graphed_adjoint', graphed_leg)
 
+    if not args.forward_only:
+      cfg['step_roofline'] = leg('step_roofline', lambda: step_roofline(
+          torch, step, state, coords, targets, ms_per_step))
+
     out['roofline'] = leg('roofline', lambda: dominant_kernel_roofline(
         torch, coords.horizontal, members, precision))
     cfg['graphed_forward_ms_per_step'] = leg('graphed_forward', lambda: graphed_forward_ms(
@@ -680,6 +698,31 @@ def main():
       out['cpu_baseline'] = leg('cpu_baseline', lambda: cpu_baseline(torch))
   finally:
     print(json.dumps(out), flush=True)
+
+
+def step_roofline(torch, step, state, coords, targets, ms_per_step):
+  """Sum over the kernels of one forward + checkpointed-adjoint step (the
+  headline's rollout, one step, with the loss) of each launch's time at its
+  binding roof -- algorithmic bytes at 8 TB/s, or flops at the FP32 MFMA /
+  FP64 peak -- against the measured time per step."""
+  from dinosaur_amd import _lib
+  with _lib.work_tally() as t:
+    rollout_fwd_adj(torch, step, state, coords, targets, 1)
+    _sync(torch)
+  roof = t.roof_seconds()
+  total = sum(roof.values())
+  fams = {k: {'launches': e['launches'], 'gbytes': e['bytes'] / 1e9,
+              'gflops': e['flops'] / 1e9, 'roof': e['roof'],
+              't_roof_ms': roof[k] * 1e3}
+          for k, e in sorted(t.families.items(), key=lambda kv: -roof[kv[0]])}
+  res = {'t_roof_ms': total * 1e3, 'wall_ms': ms_per_step, 'frac': total * 1e3 / ms_per_step,
+         'gbytes': sum(e['bytes'] for e in t.families.values()) / 1e9,
+         'families': fams,
+         'definition': 'sum over launches of max(algorithmic bytes / 8 TB/s, flops / '
+                       'peak of the binding unit) divided by the measured ms per step'}
+  print(f'[bench] step roofline: {res["t_roof_ms"]:.1f} ms at the roofs vs '
+        f'{ms_per_step:.1f} ms measured (frac {res["frac"]:.3f})', file=sys.stderr, flush=True)
+  return res
 
 
 def dominant_kernel_roofline(torch, grid, members, precision):

@@ -117,6 +117,73 @@ def skip_values() -> bool:
   return getattr(values_unread, 'active', False)
 
 
+class WorkTally:
+  """Algorithmic work of the kernels launched while active, per kernel
+  family: launches, bytes (each input read once and each output written
+  once, modal arrays counted on the triangular truncation only) and flops,
+  with the roof that binds them ('hbm', 'f32_mfma' or 'f64').  A reporting
+  aid for roofline fractions: the counts come from the call sizes, not from
+  hardware counters."""
+
+  def __init__(self):
+    self.families = {}
+
+  def add(self, family: str, nbytes: float, flops: float = 0.0, roof: str = 'hbm'):
+    e = self.families.setdefault(family, {'launches': 0, 'bytes': 0.0, 'flops': 0.0,
+                                          'roof': roof})
+    e['launches'] += 1
+    e['bytes'] += float(nbytes)
+    e['flops'] += float(flops)
+
+  # MI355X published peaks
+  PEAK = {'hbm': 8.0e12, 'f32_mfma': 157.3e12, 'f64': 78.6e12}
+
+  def roof_seconds(self) -> dict:
+    """Per family, the time at its binding roof: max(bytes / HBM peak,
+    flops / compute peak)."""
+    out = {}
+    for name, e in self.families.items():
+      t_mem = e['bytes'] / self.PEAK['hbm']
+      t_cmp = e['flops'] / self.PEAK[e['roof']] if e['roof'] != 'hbm' else 0.0
+      out[name] = max(t_mem, t_cmp)
+    return out
+
+
+_active = None
+
+
+def triangle_entries(rows: int, degrees: int) -> int:
+  """Entries of a Real-layout (2M-1, L) modal array inside the triangular
+  truncation |m| <= l (cos and sin rows of each m > 0, one row for m = 0)."""
+  m_count = (rows + 1) // 2
+  n_tri = sum(max(0, degrees - m) for m in range(m_count))
+  return 2 * n_tri - degrees
+
+
+def tally(family: str, nbytes: float, flops: float = 0.0, roof: str = 'hbm') -> None:
+  """Adds one launch to the active WorkTally, if any."""
+  t = _active
+  if t is not None:
+    t.add(family, nbytes, flops, roof)
+
+
+class work_tally:
+  """Context manager that makes a new WorkTally active for the process (the
+  backward passes of autograd run on other threads, so the tally is not
+  per thread); not re-entrant."""
+
+  def __enter__(self) -> WorkTally:
+    global _active
+    self.t = WorkTally()
+    _active = self.t
+    return self.t
+
+  def __exit__(self, *exc):
+    global _active
+    _active = None
+    return False
+
+
 def lib() -> ctypes.CDLL:
   """Loads the HIP library (raises ImportError if it was not built)."""
   global _lib

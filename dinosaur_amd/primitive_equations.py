@@ -364,6 +364,24 @@ class _ImplicitOperator:
       rc = getattr(_lib.lib(), fn_name)(f, p(div), p(temp), p(lnps), p(od), p(ot), p(ol),
                                         members, code, stream)
     _lib.check(rc, fn_name)
+    if _lib._active is not None:
+      self._tally(fn_name, members, div)
+
+  def _tally(self, fn_name, members, like):
+    """Algorithmic work of one implicit kernel: the (2K+1) fields of each
+    member read and written once on the triangle (read twice when
+    accumulating); FP64 flops of the per-degree (2K+1)^2 block for the solve
+    and of the G, R T_ref, H and dsigma products for the terms."""
+    K = self.K
+    n_real = _lib.triangle_entries(self.R, self.L)
+    cols = n_real * members
+    passes = 3 if fn_name.endswith('accumulate') else 2
+    nbytes = like.element_size() * cols * (2 * K + 1) * passes
+    if 'solve' in fn_name:
+      flops = 2.0 * (2 * K + 1) ** 2 * cols
+    else:
+      flops = 2.0 * (2 * K * K + 2 * K) * cols
+    _lib.tally(fn_name[len('dino_'):].replace('_accumulate', ''), nbytes, flops, 'f64')
 
   def apply(self, fn_name, eta, div, temp, lnps):
     div = div.contiguous()
@@ -448,6 +466,7 @@ class _NodalFused(torch.autograd.Function):
         rc = lib.dino_nodal_explicit(h.handle, ctypes.c_void_p(nodal.data_ptr()),
                                      ctypes.c_void_p(out.data_ptr()), members, code, stream)
       _lib.check(rc, 'dino_nodal_explicit')
+      _lib.tally('nodal', nodal.element_size() * nodal[0].numel() * (n_in + n_out))
     ctx.save_for_backward(nodal)
     ctx.h, ctx.members, ctx.code = h, members, code
     return out
@@ -463,6 +482,8 @@ class _NodalFused(torch.autograd.Function):
           ctx.h.handle, ctypes.c_void_p(nodal.data_ptr()), ctypes.c_void_p(g.data_ptr()),
           ctypes.c_void_p(gin.data_ptr()), ctx.members, ctx.code, stream)
     _lib.check(rc, 'dino_nodal_explicit_adjoint')
+    _lib.tally('nodal_adjoint', g.element_size() * nodal[0].numel()
+               * (2 * nodal.shape[0] + g.shape[0]))
     return gin, None, None
 
 
@@ -488,6 +509,10 @@ def _launch(name, h, a, b, out, members, like):
   with torch.cuda.device(like.device):
     rc = getattr(_lib.lib(), name)(h.handle, ptr(a), ptr(b), ptr(out), members, code, stream)
   _lib.check(rc, name)
+  if _lib._active is not None:
+    n_real = _lib.triangle_entries(*out.shape[-2:])
+    fields = a.shape[0] + (1 if b is not None else 0) + out.shape[0]
+    _lib.tally(name[len('dino_'):], like.element_size() * n_real * fields)
 
 
 class _ModalPrep(torch.autograd.Function):

@@ -215,7 +215,27 @@ def sht_call(fn_name: str, plan: Plan, x: torch.Tensor, out_tail: tuple, precisi
     rc = fn(plan.handle, _ptr(x), _ptr(out), _ptr(work), batch, _dtype_code(x), precision,
             _stream(x.device))
   _lib.check(rc, fn_name)
+  if _lib._active is not None:
+    _tally_transform(fn_name, plan, batch, x)
   return out
+
+
+def _tally_transform(fn_name: str, plan: Plan, batch: int, x: torch.Tensor):
+  """Algorithmic work of the two stages of one transform call: the Fourier
+  rows (batch, 2M-1, J) are written once and read once; modal arrays count
+  their triangular entries; the folded Legendre sums are J flops per modal
+  entry (Jh multiply-adds on each of the two fold parities)."""
+  es = x.element_size()
+  n_real = _lib.triangle_entries(plan.R, plan.L)
+  fourier, nodal = plan.R * plan.J, plan.I * plan.J
+  roof = 'f64' if x.dtype == torch.float64 else 'f32_mfma'
+  flops = batch * plan.J * n_real
+  if fn_name in ('dino_sht_inverse', 'dino_sht_forward_adjoint'):
+    _lib.tally('legendre_inverse', es * batch * (n_real + fourier), flops, roof)
+    _lib.tally('dft_inverse', es * batch * (fourier + nodal))
+  else:
+    _lib.tally('dft_forward', es * batch * (nodal + fourier))
+    _lib.tally('legendre_forward', es * batch * (fourier + n_real), flops, roof)
 
 
 class _ToNodal(torch.autograd.Function):

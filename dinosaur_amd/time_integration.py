@@ -214,6 +214,19 @@ def _code(t):
   return _lib.DINO_F64 if t.dtype == torch.float64 else _lib.DINO_F32
 
 
+def _tally_elementwise(family, ops, like, n_full, n_range):
+  """Algorithmic bytes of a flat elementwise pass: n_full full-length and
+  n_range implicit-range operands (inputs and output), each read or written
+  once on the triangular entries of its fields."""
+  from . import _lib  # pylint: disable=g-import-not-at-top
+  frac = 1.0
+  if ops.degrees:
+    rows = ops.field_size // ops.degrees
+    frac = _lib.triangle_entries(rows, ops.degrees) / ops.field_size
+  elems = n_full * ops.n + n_range * (ops.ie - ops.ib)
+  _lib.tally(family, like.element_size() * elems * frac)
+
+
 def _fused_stage(ops, y0, dt, ex, im, scale=None):
   """y0 + dt sum(a f) + dt sum(b g) with compact implicit terms g (one pass),
   times the per-degree `scale` (a step filter) when given."""
@@ -232,6 +245,8 @@ def _fused_stage(ops, y0, dt, ex, im, scale=None):
     rc = _lib.lib().dino_imex_stage_ranged_scaled(
         *args, ctypes.c_void_p(scale.data_ptr()), scale.numel(), _code(y0), stream)
     _lib.check(rc, 'dino_imex_stage_ranged_scaled')
+  if _lib._active is not None:
+    _tally_elementwise('stage', ops, y0, 2 + len(fs), len(gs))
   return out
 
 
@@ -253,6 +268,8 @@ def _fused_stage_implicit_range(ops, y0, dt, ex, im):
       ctypes.c_void_p(sub(out).data_ptr()), n, 0, n, 0, ops.field_size, ops.degrees,
       _code(y0), stream)
   _lib.check(rc, 'dino_imex_stage_ranged')
+  if _lib._active is not None:
+    _tally_elementwise('stage', ops, y0, 0, 2 + len(fs) + len(gs))
   return out
 
 
@@ -281,6 +298,9 @@ def _fused_gather(ops, sources, dt, like, ranged=False, scale=None):
       ctypes.c_void_p(scale.data_ptr() if scale is not None else 0),
       scale.numel() if scale is not None else 0, _code(like), stream)
   _lib.check(rc, 'dino_imex_adjoint_gather')
+  if _lib._active is not None:
+    n_range = sum(1 for _, _, _, _, r in sources if r) + (1 if ranged else 0)
+    _tally_elementwise('gather', ops, like, len(sources) + 1 - n_range, n_range)
   return out
 
 

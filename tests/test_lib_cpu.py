@@ -62,3 +62,30 @@ def test_autotune_batch_buckets_and_lookup():
   assert autotune.lookup(tuned, 10 ** 6, 0, 0) == {0: 2}   # nearest bucket
   assert autotune.lookup(tuned, 100, 0, 2) == {0: 3}       # other precision
   assert autotune.lookup(tuned, 100, 1, 0) is None         # nothing tuned for FP64
+
+
+@pytest.mark.parametrize('rows,degrees', [(43, 23), (511, 257), (85, 44), (1, 1)])
+def test_triangle_entries_counts_mask(rows, degrees):
+  """Entries with |m| <= l of a Real-layout (2M-1, L) array, m order
+  0, +1, -1, +2, -2, ...; matches the SURVEY cheat-sheet's N_real."""
+  m = np.array([0] + [k for j in range(1, (rows + 1) // 2) for k in (j, j)])
+  l = np.arange(degrees)
+  assert _lib.triangle_entries(rows, degrees) == int((m[:, None] <= l[None, :]).sum())
+
+
+def test_work_tally_roofs():
+  """Time at the binding roof per family: bytes at 8 TB/s or flops at the
+  unit's peak, whichever is longer; nothing is recorded outside the context."""
+  _lib.tally('x', 1e9)
+  with _lib.work_tally() as t:
+    _lib.tally('leg', 8e9, 157.3e12 * 2e-3, 'f32_mfma')
+    _lib.tally('leg', 8e9, 0.0, 'f32_mfma')
+    _lib.tally('copy', 16e9)
+    _lib.tally('solve', 0.8e9, 78.6e12 * 1e-4, 'f64')
+  _lib.tally('x', 1e9)
+  r = t.roof_seconds()
+  assert set(r) == {'leg', 'copy', 'solve'}
+  assert t.families['leg']['launches'] == 2
+  assert abs(r['leg'] - 2e-3) < 1e-12          # 16 GB = 2 ms, 2 ms of MFMA
+  assert abs(r['copy'] - 2e-3) < 1e-12
+  assert abs(r['solve'] - 1e-4) < 1e-12        # 0.1 ms of memory, 0.1 ms of FP64

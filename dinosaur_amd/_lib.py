@@ -95,7 +95,16 @@ SIGNATURES = {
     'dino_modal_prep_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
     'dino_modal_epilogue': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
     'dino_modal_epilogue_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
+    'dino_modal_create_ex': (_c_i, [_c_i, _c_i, _c_i, _c_i, _c_i, _c_d, _c_d, _c_dp, _c_dp, _c_dp,
+                                    _c_dp, _c_dp, _c_dp, _c_dp, _c_i, ctypes.POINTER(_c_p)]),
+    'dino_nodal_hybrid_create': (_c_i, [_c_i, _c_i, _c_i, _c_i, _c_i, ctypes.POINTER(_c_i), _c_i,
+                                        _c_i, _c_dp, _c_dp, _c_dp, _c_dp, _c_dp, _c_d, _c_d, _c_d,
+                                        _c_d, _c_d, _c_d, ctypes.POINTER(_c_p)]),
+    'dino_nodal_hybrid_destroy': (_c_i, [_c_p]),
+    'dino_nodal_hybrid_fields': (_c_i64, [_c_p, _c_i64, _c_i]),
+    'dino_nodal_hybrid_explicit': (_c_i, [_c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
 }
+MODAL_HYBRID = 1
 
 _lib = None
 _lock = threading.Lock()

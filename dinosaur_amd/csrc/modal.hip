@@ -30,6 +30,7 @@
 
 struct dino_modal {
   int R, L, K, nt, hum;
+  int hybrid;  // DINO_MODAL_HYBRID: prep also emits ln ps, no humidity extras
   double radius, gravity;
   // device constant block: il[L], ev[L], mask[L], UP, DN, UP2, DN2 (R*L each),
   // jr[R] (coefficient), pr[R] (partner row, stored as a number)
@@ -67,7 +68,7 @@ __host__ __device__ inline MOff moff(int R, int L) {
 constexpr int MAXTR = 6;
 
 struct PrepGroups {  // inverse-transform input (fields)
-  int64_t z, d, t, tr[MAXTR], u, v, gx, gy, lp, qx, qy, n;
+  int64_t z, d, t, tr[MAXTR], u, v, gx, gy, lp, qx, qy, ln, n;
 };
 struct StateGroups {  // state leaves in one buffer: vort, div, temp, lnps, tracers
   int64_t z, d, t, lnps, tr[MAXTR], n;
@@ -76,7 +77,9 @@ struct EpiGroups {  // forward-transform output (fields)
   int64_t cu, cv, ke, te, ut, vt, lsp, trn[MAXTR], tu[MAXTR], tv[MAXTR], curl, dc, n;
 };
 
-inline PrepGroups prep_groups(int K, int nt, bool moist, int64_t E) {
+// hybrid: the sigma groups of a dry state plus ln ps itself (LN), which the
+// hybrid nodal kernel needs for the column's surface pressure
+inline PrepGroups prep_groups(int K, int nt, bool moist, int64_t E, bool hybrid = false) {
   PrepGroups g;
   std::memset(&g, 0, sizeof(g));
   int64_t o = 0;
@@ -85,6 +88,7 @@ inline PrepGroups prep_groups(int K, int nt, bool moist, int64_t E) {
   for (int i = 0; i < nt; ++i) g.tr[i] = take(K);
   g.u = take(K); g.v = take(K); g.gx = take(1); g.gy = take(1);
   if (moist) { g.lp = take(1); g.qx = take(K); g.qy = take(K); }
+  if (hybrid) g.ln = take(1);
   g.n = o;
   return g;
 }
@@ -114,7 +118,8 @@ inline EpiGroups epi_groups(int K, int nt, bool moist, int64_t E) {
 }
 
 struct MP {
-  int R, L, K, nt, hum;
+  int R, L, K, nt, hum;  // hum < 0 also on hybrid levels (no humidity extras)
+  int hybrid;
   int kc, nkc;  // levels per thread and level chunks: blockIdx.y = member * nkc + chunk
   int64_t RL;
   double a, g;
@@ -249,6 +254,7 @@ __global__ __launch_bounds__(256) void prep_kernel(const T* __restrict__ st, T* 
   *(W(out, pg.gx, 1, 0) + pos) = S.dlon(lp) * inva;
   *(W(out, pg.gy, 1, 0) + pos) = S.cldl(lp) * inva;
   if (p.hum >= 0) *(W(out, pg.lp, 1, 0) + pos) = lp[pos] * cst[o.ev + l];
+  if (p.hybrid) *(W(out, pg.ln, 1, 0) + pos) = lp[pos];
 }
 
 template <typename T>
@@ -296,6 +302,7 @@ __global__ __launch_bounds__(256) void prep_adjoint_kernel(const T* __restrict__
   if (kch != 0) return;  // the single-level field, once per member
   T lb = (-S.dlon(F(gout, pg.gx, 1, 0)) + S.cldl_t(F(gout, pg.gy, 1, 0))) * inva;
   if (p.hum >= 0) lb += cst[o.ev + l] * *(F(gout, pg.lp, 1, 0) + pos);
+  if (p.hybrid) lb += *(F(gout, pg.ln, 1, 0) + pos);
   *(W(gst, sg.lnps, 1, 0) + pos) = lb;
 }
 
@@ -435,7 +442,8 @@ MP make_mp(const dino_modal* m) {
   p.L = m->L;
   p.K = m->K;
   p.nt = m->nt;
-  p.hum = m->hum;
+  p.hum = m->hybrid ? -1 : m->hum;
+  p.hybrid = m->hybrid;
   p.kc = DINO_MODAL_LEVELS_PER_THREAD < m->K ? DINO_MODAL_LEVELS_PER_THREAD : m->K;
   p.nkc = (m->K + p.kc - 1) / p.kc;
   p.RL = (int64_t)m->R * m->L;
@@ -453,7 +461,7 @@ int run(int which, const dino_modal* m, const void* a, const void* b, void* out,
     p.kc *= 2;
     p.nkc = (m->K + p.kc - 1) / p.kc;
   }
-  bool moist = m->hum >= 0;
+  const bool moist = m->hum >= 0 && !m->hybrid, hyb = m->hybrid != 0;
   StateGroups sg = state_groups(m->K, m->nt, E);
   const unsigned gy = (unsigned)(E * p.nkc);
   dim3 grid((unsigned)((p.RL + 255) / 256), gy);
@@ -462,11 +470,11 @@ int run(int which, const dino_modal* m, const void* a, const void* b, void* out,
   switch (which) {
     case 0:
       prep_kernel<T><<<tgrid, 256, 0, s>>>((const T*)a, (T*)out, c, m->tile_pos, m->n_tile_pos,
-                                           sg, prep_groups(m->K, m->nt, moist, E), p);
+                                           sg, prep_groups(m->K, m->nt, moist, E, hyb), p);
       break;
     case 1:
       prep_adjoint_kernel<T><<<grid, 256, 0, s>>>((const T*)a, (T*)out, c, sg,
-                                                  prep_groups(m->K, m->nt, moist, E), p);
+                                                  prep_groups(m->K, m->nt, moist, E, hyb), p);
       break;
     case 2:
       epilogue_kernel<T><<<grid, 256, 0, s>>>((const T*)a, (const T*)b, (T*)out, c,
@@ -486,12 +494,14 @@ int run(int which, const dino_modal* m, const void* a, const void* b, void* out,
 
 using namespace dino;
 
-extern "C" int dino_modal_create(int R, int L, int K, int n_tracers, int humidity_index,
-                                 double radius, double gravity, const double* inv_lap,
-                                 const double* lap, const double* clip_mask, const double* up,
-                                 const double* dn, const double* up2, const double* dn2,
-                                 dino_modal** out) {
+extern "C" int dino_modal_create_ex(int R, int L, int K, int n_tracers, int humidity_index,
+                                    double radius, double gravity, const double* inv_lap,
+                                    const double* lap, const double* clip_mask, const double* up,
+                                    const double* dn, const double* up2, const double* dn2,
+                                    int flags, dino_modal** out) {
   DINO_REQUIRE(out, DINO_EINVAL, "dino_modal_create: out is NULL");
+  DINO_REQUIRE((flags & ~DINO_MODAL_HYBRID) == 0, DINO_EINVAL, "unknown modal flags 0x%x",
+               flags);
   *out = nullptr;
   DINO_REQUIRE(R >= 1 && R % 2 == 1 && L >= 2 && K >= 1 && (int64_t)R * L < (int64_t)1 << 31,
                DINO_EINVAL, "invalid modal shape (%d, %d) x %d levels", R, L, K);
@@ -537,6 +547,7 @@ extern "C" int dino_modal_create(int R, int L, int K, int n_tracers, int humidit
   m->K = K;
   m->nt = n_tracers;
   m->hum = humidity_index;
+  m->hybrid = (flags & DINO_MODAL_HYBRID) ? 1 : 0;
   m->radius = radius;
   m->gravity = gravity;
   hipError_t e = hipMalloc(&m->c64, sizeof(double) * o.total);
@@ -558,6 +569,15 @@ extern "C" int dino_modal_create(int R, int L, int K, int n_tracers, int humidit
   return DINO_OK;
 }
 
+extern "C" int dino_modal_create(int R, int L, int K, int n_tracers, int humidity_index,
+                                 double radius, double gravity, const double* inv_lap,
+                                 const double* lap, const double* clip_mask, const double* up,
+                                 const double* dn, const double* up2, const double* dn2,
+                                 dino_modal** out) {
+  return dino_modal_create_ex(R, L, K, n_tracers, humidity_index, radius, gravity, inv_lap, lap,
+                              clip_mask, up, dn, up2, dn2, 0, out);
+}
+
 extern "C" int dino_modal_destroy(dino_modal* m) {
   if (m) {
     if (m->c64) (void)hipFree(m->c64);
@@ -570,9 +590,9 @@ extern "C" int dino_modal_destroy(dino_modal* m) {
 
 extern "C" int64_t dino_modal_fields(const dino_modal* m, int64_t members, int which) {
   if (!m || members < 0) return -1;
-  bool moist = m->hum >= 0;
+  const bool moist = m->hum >= 0 && !m->hybrid;
   if (which == 0) return state_groups(m->K, m->nt, members).n;
-  if (which == 1) return prep_groups(m->K, m->nt, moist, members).n;
+  if (which == 1) return prep_groups(m->K, m->nt, moist, members, m->hybrid != 0).n;
   return epi_groups(m->K, m->nt, moist, members).n;
 }
 

@@ -302,17 +302,28 @@ def _vadv(w, x, dsig_c2c_inv, dtype_like):
 class _ImplicitOperator:
   """Holds the FP64 level matrices and per-eta GPU factor objects."""
 
-  def __init__(self, coords, reference_temperature, kappa, r_gas):
+  def __init__(self, coords, reference_temperature, kappa, r_gas, matrices=None):
+    """matrices: None for sigma levels, else (G, H, c, w) of another vertical
+    discretisation whose implicit block matrix has the same pattern
+    [[I, eta lam G, eta lam c], [eta H, I, 0], [eta w^T, 0, 1]]: c takes the
+    place of R T_ref and w that of dsigma."""
     v = coords.vertical
     self.K = v.layers
     self.L = coords.horizontal.modal_shape[1]
     self.R = coords.horizontal.modal_shape[0]
     self.r_gas = float(r_gas)
-    self.G = np.ascontiguousarray(get_geopotential_weights_sigma(v, r_gas), np.float64)
-    self.H = np.ascontiguousarray(
-        get_temperature_implicit_weights_sigma(v, reference_temperature, kappa), np.float64)
-    self.tref = np.ascontiguousarray(np.asarray(reference_temperature, np.float64))
-    self.dsig = np.ascontiguousarray(v.layer_thickness.astype(np.float64))
+    if matrices is None:
+      G = get_geopotential_weights_sigma(v, r_gas)
+      H = get_temperature_implicit_weights_sigma(v, reference_temperature, kappa)
+      tref = np.asarray(reference_temperature, np.float64)
+      dsig = v.layer_thickness
+    else:
+      G, H, c, dsig = matrices
+      tref = np.asarray(c, np.float64) / self.r_gas
+    self.G = np.ascontiguousarray(G, np.float64)
+    self.H = np.ascontiguousarray(H, np.float64)
+    self.tref = np.ascontiguousarray(tref, np.float64)
+    self.dsig = np.ascontiguousarray(np.asarray(dsig, np.float64))
     self.lam = np.ascontiguousarray(coords.horizontal.laplacian_eigenvalues.astype(np.float64))
     a = self.G @ self.H + self.r_gas * np.outer(self.tref, self.dsig)
     d, vecs = np.linalg.eig(a)
@@ -983,12 +994,17 @@ class PrimitiveEquationsSigma:
     return State.from_flat(tend, lead, K, names,
                            sim_time=None if state.sim_time is None else 1.0)
 
+  @property
+  def _modal_moist(self) -> bool:
+    """Whether the humidity curl/divergence terms enter the modal assembly."""
+    return self.humidity_key is not None
+
   def _explicit_unfused(self, state: State, names) -> State:
     grid = self.coords.horizontal
     ps = self.physics_specs
     a = grid.radius
     x = state.vorticity
-    moist = self.humidity_key is not None
+    moist = self._modal_moist
     inv = self._inverse_inputs(state, names)
     nod = grid.to_nodal_batched(inv)
     mod = grid.to_modal_batched(self._nodal_unfused(nod, names, x))
@@ -1137,3 +1153,395 @@ class MoistPrimitiveEquationsWithCloudMoisture(PrimitiveEquationsSigma):
                      humidity_key='specific_humidity',
                      cloud_keys=('specific_cloud_liquid_water_content',
                                  'specific_cloud_ice_water_content'))
+
+
+# ------------------------------------------------------------------ hybrid levels
+
+
+def _hybrid_coefficients(a, b, ps, xp=np):
+  """Simmons-Burridge coefficients on interfaces p = a + b ps
+  (primitive_equations.py:1483-1533): alpha_k = 1 - p_{k-1/2} / dp_k *
+  ln(p_{k+1/2} / p_{k-1/2}) and that logarithm, pressures floored at 1e-6 ps
+  (the model top has p = 0).  ps: a scalar (a, b 1-D) or a (..., 1, I, J)
+  tensor (a, b (K+1, 1, 1) tensors)."""
+  p_half = a + b * ps
+  if xp is np:
+    upper, lower = p_half[:-1], p_half[1:]
+    floor = 1e-6 * ps
+    log_ratio = np.log(np.maximum(lower, floor) / np.maximum(upper, floor))
+  else:
+    upper, lower = p_half[..., :-1, :, :], p_half[..., 1:, :, :]
+    floor = 1e-6 * ps
+    log_ratio = torch.log(torch.maximum(lower, floor) / torch.maximum(upper, floor))
+  dp = lower - upper
+  return 1 - (upper / dp) * log_ratio, log_ratio
+
+
+def get_geopotential_weights_hybrid(coordinates, ideal_gas_constant, p_s_ref) -> np.ndarray:
+  """R [alpha_k on the diagonal, ln(p ratio)_j at j > k], the coefficients at
+  the reference surface pressure (primitive_equations.py:1605-1626)."""
+  alpha, log_ratio = _hybrid_coefficients(coordinates.a_boundaries, coordinates.b_boundaries,
+                                          p_s_ref)
+  k = coordinates.layers
+  w = np.triu(np.ones((k, k)), 1) * log_ratio[None, :] + np.diag(alpha)
+  return ideal_gas_constant * w
+
+
+def get_temperature_implicit_weights_hybrid(coordinates, reference_temperature, kappa,
+                                            p_s_ref) -> np.ndarray:
+  """H of the hybrid implicit temperature tendency (primitive_equations.py:
+  1736-1783): kappa T_ref times alpha-weighted lower-triangular sums over
+  dp_ref (rows), minus the reference-lapse advection (driven by sigma
+  thicknesses), times dp_ref of the divergence level (columns)."""
+  tref = np.asarray(reference_temperature)
+  if tref.ndim != 1 or tref.shape[-1] != coordinates.layers:
+    raise ValueError('`reference_temp` must be a vector of length `coordinates.layers`; '
+                     f'got shape {tref.shape} and {coordinates.layers} layers.')
+  k = coordinates.layers
+  lower = np.tril(np.ones((k, k)))
+  alpha, _ = _hybrid_coefficients(coordinates.a_boundaries, coordinates.b_boundaries, p_s_ref)
+  pa = lower * alpha[:, None]
+  pa_above = np.concatenate([np.zeros((1, k)), pa[:-1]], axis=0)
+  dp_ref = coordinates.pressure_thickness + coordinates.sigma_thickness * p_s_ref
+  h0 = kappa * tref[:, None] * (pa + pa_above) / dp_ref[:, None]
+  lapse = np.append(np.diff(tref) / (dp_ref[:-1] + dp_ref[1:]), 0.0)[:, None]
+  adv = lapse * (lower - np.cumsum(coordinates.sigma_thickness)[:, None])
+  adv_above = np.concatenate([np.zeros((1, k)), adv[:-1]], axis=0)
+  # the last factor scales columns (the divergence level j), as the reference's
+  # broadcast of the 1-D thickness does
+  return (h0 - adv - adv_above) * dp_ref[None, :]
+
+
+def _pgf_lps_reference(coordinates, reference_temperature, ideal_gas_constant, p_s_ref):
+  """Coefficient of grad ln ps in the pressure-gradient force at the
+  reference state, R T_ref / dp [ln(p ratio) b_{k-1/2} + alpha db] ps_ref
+  (primitive_equations.py:1859-1888)."""
+  alpha, log_ratio = _hybrid_coefficients(coordinates.a_boundaries, coordinates.b_boundaries,
+                                          p_s_ref)
+  dp = np.maximum(coordinates.layer_thickness(p_s_ref), 1e-6 * p_s_ref)
+  weight = log_ratio * coordinates.b_boundaries[:-1] + alpha * coordinates.sigma_thickness
+  return ideal_gas_constant * np.asarray(reference_temperature, np.float64) / dp * weight * p_s_ref
+
+
+def hybrid_vertical_advection(mass_flux, x, pressure_thickness):
+  """-(M_{k+1/2} (x_{k+1} - x_k) + M_{k-1/2} (x_k - x_{k-1})) / (2 dp_k), the
+  mass flux given at the K - 1 internal interfaces (primitive_equations.py:
+  1941-1957, whose flux array also carries the zero end values)."""
+  flux = mass_flux * torch.diff(x, dim=-3)
+  padded = torch.nn.functional.pad(flux, (0, 0, 0, 0, 1, 1))
+  return -0.5 * (padded[..., 1:, :, :] + padded[..., :-1, :, :]) / pressure_thickness
+
+
+def get_geopotential_diff_hybrid(temperature, coordinates, ideal_gas_constant, surface_pressure):
+  """R [sum_{j>k} T_j ln(p ratio)_j + alpha_k T_k] with the coefficients of
+  the nodal surface pressure (primitive_equations.py:1629-1674); `coordinates`
+  in the units of `surface_pressure`."""
+  like = temperature
+  vec = lambda v: torch.as_tensor(np.asarray(v, np.float64), device=like.device).to(
+      like.dtype).reshape(-1, 1, 1)
+  alpha, log_ratio = _hybrid_coefficients(vec(coordinates.a_boundaries),
+                                          vec(coordinates.b_boundaries), surface_pressure, torch)
+  w = temperature * log_ratio
+  below = torch.flip(torch.cumsum(torch.flip(w, (-3,)), dim=-3), (-3,)) - w
+  return ideal_gas_constant * (below + alpha * temperature)
+
+
+def get_geopotential_on_hybrid(temperature, surface_pressure, specific_humidity=None,
+                               clouds=None, *, nodal_orography, coordinates,
+                               gravity_acceleration, ideal_gas_constant,
+                               water_vapor_gas_constant=None):
+  """Nodal geopotential from nodal temperature (and moisture) on hybrid levels
+  (primitive_equations.py:1677-1733)."""
+  if specific_humidity is not None:
+    if water_vapor_gas_constant is None:
+      raise ValueError('Must provide `water_vapor_gas_constant` with `specific_humidity`.')
+    ratio = water_vapor_gas_constant / ideal_gas_constant
+    cloud = 0.0 if clouds is None else clouds
+    temperature = temperature * (1 + (ratio - 1) * specific_humidity - cloud)
+  return nodal_orography * gravity_acceleration + get_geopotential_diff_hybrid(
+      temperature, coordinates, ideal_gas_constant, surface_pressure)
+
+
+class _NodalHybridFused(torch.autograd.Function):
+  """(fields_in, I, J) nodal inputs -> (fields_out, I, J) products on hybrid
+  levels, by the fused HIP kernel; the backward differentiates the same terms
+  written as tensor operations (PrimitiveEquationsHybrid._nodal_terms)."""
+
+  @staticmethod
+  def forward(ctx, nodal, h, members, eq, names):
+    nodal = nodal.contiguous()
+    lib = _lib.lib()
+    n_out = lib.dino_nodal_hybrid_fields(h.handle, members, 1)
+    n_in = lib.dino_nodal_hybrid_fields(h.handle, members, 0)
+    if nodal.shape[0] != n_in:
+      raise ValueError(f'hybrid nodal kernel expects {n_in} input fields, got {nodal.shape[0]}')
+    out = torch.empty((n_out,) + tuple(nodal.shape[1:]), dtype=nodal.dtype, device=nodal.device)
+    code = _lib.DINO_F64 if nodal.dtype == torch.float64 else _lib.DINO_F32
+    stream = ctypes.c_void_p(torch.cuda.current_stream(nodal.device).cuda_stream)
+    if not _lib.skip_values():  # the backward reads only the saved inputs
+      with torch.cuda.device(nodal.device):
+        rc = lib.dino_nodal_hybrid_explicit(h.handle, ctypes.c_void_p(nodal.data_ptr()),
+                                            ctypes.c_void_p(out.data_ptr()), members, code, stream)
+      _lib.check(rc, 'dino_nodal_hybrid_explicit')
+      _lib.tally('nodal_hybrid', nodal.element_size() * nodal[0].numel() * (n_in + n_out))
+    ctx.save_for_backward(nodal)
+    ctx.members, ctx.eq, ctx.names = members, eq, names
+    return out
+
+  @staticmethod
+  def backward(ctx, g):
+    (nodal,) = ctx.saved_tensors
+    with torch.enable_grad():
+      x = nodal.detach().requires_grad_(True)
+      y = ctx.eq._nodal_buffer_terms(x, ctx.members, ctx.names)
+      (gin,) = torch.autograd.grad(y, x, g)
+    return gin, None, None, None, None
+
+
+@dataclasses.dataclass
+class PrimitiveEquationsHybrid(PrimitiveEquationsSigma):
+  """Primitive equations on hybrid sigma-pressure levels
+  (primitive_equations.py:1960-2518).
+
+  coords.vertical is a HybridCoordinates with a in hPa (`hpa_quantity`);
+  reference_surface_pressure is a pressure quantity (default 101325 Pa).
+  Explicit terms: the modal prep kernel, one inverse transform (which also
+  carries ln ps, for the columns' surface pressure), the fused hybrid nodal
+  kernel, one forward transform and the sigma modal epilogue.  Implicit terms
+  and inverse: the FP64 vertical-mode kernels of the sigma equations with
+  the hybrid matrices (the block pattern of the implicit operator is the
+  same, :1890-1938).  Moisture enters through the virtual temperature only
+  (the sigma humidity tendencies are zero here, :2491-2518).
+  """
+  reference_surface_pressure: Any = dataclasses.field(
+      default=101325.0 * units_lib.scales.units.pascal, kw_only=True)
+  hpa_quantity: Any = dataclasses.field(default=units_lib.scales.units.hPa, kw_only=True)
+
+  def __post_init__(self):
+    super().__post_init__()
+    ps = self.physics_specs
+    p_ref = ps.nondimensionalize(self.reference_surface_pressure)
+    if p_ref <= 0:
+      raise ValueError('`reference_surface_pressure` must be positive.')
+    self._p_s_ref = float(p_ref)
+    from . import hybrid_coordinates  # pylint: disable=import-outside-toplevel
+    v = self.coords.vertical
+    if not isinstance(v, hybrid_coordinates.HybridCoordinates):
+      raise TypeError(f'PrimitiveEquationsHybrid needs HybridCoordinates, got {type(v)}')
+    a = np.asarray(ps.nondimensionalize(self.hpa_quantity * np.asarray(v.a_boundaries,
+                                                                        np.float64)))
+    self.nondim_levels = hybrid_coordinates.HybridCoordinates(a, np.asarray(v.b_boundaries,
+                                                                            np.float64))
+    self.nondim_coords = coordinate_systems.CoordinateSystem(self.coords.horizontal,
+                                                             self.nondim_levels)
+
+  @property
+  def p_s_ref(self) -> float:
+    return self._p_s_ref
+
+  @property
+  def _modal_moist(self) -> bool:
+    return False
+
+  @functools.cached_property
+  def _implicit(self) -> _ImplicitOperator:
+    lev = self.nondim_levels
+    ps = self.physics_specs
+    G = get_geopotential_weights_hybrid(lev, ps.R, self.p_s_ref)
+    H = get_temperature_implicit_weights_hybrid(lev, self.reference_temperature, ps.kappa,
+                                                self.p_s_ref)
+    c = _pgf_lps_reference(lev, self.reference_temperature, ps.R, self.p_s_ref)
+    w = lev.pressure_thickness / self.p_s_ref + lev.sigma_thickness
+    return _ImplicitOperator(self.coords, self.reference_temperature, ps.kappa, ps.R,
+                             matrices=(G, H, c, w))
+
+  # ---- explicit terms
+  def _nodal_handle(self, device, tracer_names):
+    key = (torch.device(device).index, tuple(tracer_names))
+    cache = self.__dict__.setdefault('_nodal_handles', {})
+    h = cache.get(key)
+    if h is not None:
+      return h
+    grid = self.coords.horizontal
+    lev = self.nondim_levels
+    ps = self.physics_specs
+    f64 = lambda a: np.ascontiguousarray(np.asarray(a, np.float64))
+    hum = tracer_names.index(self.humidity_key) if self.humidity_key is not None else -1
+    clouds = [tracer_names.index(k) for k in (self.cloud_keys or ())]
+    cl = (ctypes.c_int * 4)(*(clouds + [0] * (4 - len(clouds))))
+    handle = ctypes.c_void_p()
+    dp = _lib.dptr
+    sec2 = f64(grid.sec2_lat)
+    cor = f64(2 * ps.angular_velocity * grid.nodal_axes[1])
+    a, b, tref = f64(lev.a_boundaries), f64(lev.b_boundaries), f64(self.reference_temperature)
+    with torch.cuda.device(device):
+      rc = _lib.lib().dino_nodal_hybrid_create(
+          lev.layers, grid.longitude_nodes, grid.latitude_nodes, len(tracer_names), hum, cl,
+          len(clouds), int(self.include_vertical_advection), dp(sec2), dp(cor), dp(a), dp(b),
+          dp(tref), float(self.p_s_ref), float(ps.R), float(ps.kappa), float(ps.R_vapor),
+          float(ps.Cp), float(ps.Cp_vapor), ctypes.byref(handle))
+    _lib.check(rc, 'dino_nodal_hybrid_create')
+    h = _Handle(handle, 'dino_nodal_hybrid_destroy')
+    cache[key] = h
+    return h
+
+  def _modal_handle(self, device, names):
+    key = (torch.device(device).index, tuple(names))
+    cache = self.__dict__.setdefault('_modal_handles', {})
+    h = cache.get(key)
+    if h is not None:
+      return h
+    grid = self.coords.horizontal
+    R_, L_ = grid.modal_shape
+    f64 = lambda a: np.ascontiguousarray(np.asarray(a, np.float64))
+    il = f64(spherical_harmonic._grid_host_const(grid, 'inverse_laplacian'))
+    ev = f64(grid.laplacian_eigenvalues)
+    mask = np.ones(L_)
+    mask[-1] = 0.0
+    cst = [f64(spherical_harmonic._grid_host_const(grid, n))
+           for n in ('cdlat_up', 'cdlat_dn', 'scdlat_up', 'scdlat_dn')]
+    hum = names.index(self.humidity_key) if self.humidity_key is not None else -1
+    handle = ctypes.c_void_p()
+    dp = _lib.dptr
+    with torch.cuda.device(device):
+      rc = _lib.lib().dino_modal_create_ex(
+          R_, L_, self.coords.vertical.layers, len(names), hum, float(grid.radius),
+          float(self.physics_specs.g), dp(il), dp(ev), dp(f64(mask)), *(dp(c) for c in cst),
+          _lib.MODAL_HYBRID, ctypes.byref(handle))
+    _lib.check(rc, 'dino_modal_create_ex')
+    h = _Handle(handle, 'dino_modal_destroy')
+    cache[key] = h
+    return h
+
+  def _explicit_fused(self, state: State, names) -> State:
+    """As the sigma path, with the hybrid nodal kernel between the transforms."""
+    sh = self.coords.horizontal.spherical_harmonics
+    flat = to_flat(state)
+    buf = flat.flat_buffer()
+    lead, K, _ = flat._flat[1:]
+    members = int(np.prod(lead)) if lead else 1
+    mh = self._modal_handle(buf.device, names)
+    prep = _ModalPrep.apply(buf, mh, members)
+    own = (type(sh).inverse_transform is spherical_harmonic.RealSphericalHarmonics.inverse_transform
+           and type(sh).transform is spherical_harmonic.RealSphericalHarmonics.transform)
+    kw_inv = {'_triangle_only_adjoint': True} if own else {}
+    kw_fwd = {'_triangle_only': True} if own else {}
+    nodal = sh.inverse_transform(prep, **kw_inv)
+    nout = _NodalHybridFused.apply(nodal, self._nodal_handle(buf.device, names), members, self,
+                                   tuple(names))
+    mod = sh.transform(nout, **kw_fwd)
+    tend = _ModalEpilogue.apply(mod, mh, members, self._orography_modal(buf).contiguous())
+    return State.from_flat(tend, lead, K, names,
+                           sim_time=None if state.sim_time is None else 1.0)
+
+  def _inverse_inputs(self, state, names):
+    """The dry sigma inputs plus ln ps itself (order of the nodal kernel's
+    input groups)."""
+    grid = self.coords.horizontal
+    a = grid.radius
+    zeta, delta = state.vorticity, state.divergence
+    lnps = state.log_surface_pressure
+    psi = grid.inverse_laplacian(zeta)
+    chi = grid.inverse_laplacian(delta)
+    ucos_m = (grid.d_dlon(chi) - grid.cos_lat_d_dlat(psi)) / a
+    vcos_m = (grid.cos_lat_d_dlat(chi) + grid.d_dlon(psi)) / a
+    inv = [zeta, delta, state.temperature_variation] + [state.tracers[k] for k in names]
+    inv += [ucos_m, vcos_m, grid.d_dlon(lnps) / a, grid.cos_lat_d_dlat(lnps) / a, lnps]
+    return inv
+
+  def _nodal_unfused(self, nod, names, x):
+    i0 = 3 + len(names)
+    return self._nodal_terms(nod[0], nod[1], nod[2], dict(zip(names, nod[3:i0])), *nod[i0:i0 + 5])
+
+  def _nodal_buffer_terms(self, nodal, members, names):
+    """_nodal_terms on a (fields, I, J) buffer with the fused kernel's groups;
+    returns the kernel's output buffer."""
+    E, K = members, self.coords.vertical.layers
+    I, J = nodal.shape[-2:]
+    sizes = [K] * (3 + len(names)) + [K, K, 1, 1, 1]
+    parts = torch.split(nodal, [E * n for n in sizes])
+    parts = [t.reshape(E, n, I, J) for t, n in zip(parts, sizes)]
+    nt = len(names)
+    outs = self._nodal_terms(parts[0], parts[1], parts[2], dict(zip(names, parts[3:3 + nt])),
+                             *parts[3 + nt:])
+    return torch.cat([o.reshape(-1, I, J) for o in outs])
+
+  def _nodal_terms(self, zn, dn, tn, trn, un, vn, gxn, gyn, lnn):
+    """Nodal products of the hybrid explicit terms as tensor operations, in
+    the fused kernel's output-group order (primitive_equations.py:1536-1602,
+    2032-2362): CU CV KE TEMP UT VT LSP (TRN TU TV) per tracer."""
+    ps_ = self.physics_specs
+    lev = self.nondim_levels
+    K = lev.layers
+    like = tn
+    vec = lambda v: torch.as_tensor(np.asarray(v, np.float64), device=like.device).to(
+        like.dtype).reshape(-1, 1, 1)
+    sec2 = self._c('sec2', like)
+    cor = self._c('coriolis', like)
+    tref = self._c('T_ref', like)
+    ah, bh = vec(lev.a_boundaries), vec(lev.b_boundaries)
+    da, db = vec(lev.pressure_thickness), vec(lev.sigma_thickness)
+    aref_np, lref_np = _hybrid_coefficients(lev.a_boundaries, lev.b_boundaries, self.p_s_ref)
+    aref, lref = vec(aref_np), vec(lref_np)
+    pref = vec(_pgf_lps_reference(lev, self.reference_temperature, ps_.R, self.p_s_ref))
+    ps = torch.exp(lnn)
+    dp = da + db * ps
+    alpha, lr = _hybrid_coefficients(ah, bh, ps, torch)
+    udg = (un * gxn + vn * gyn) * sec2
+    adv = db * ps * udg
+    PA = torch.cumsum(adv, dim=-3)
+    PF = torch.cumsum(dp * dn + adv, dim=-3)
+    b_int = bh[1:K]
+    m_full = b_int * PF[..., -1:, :, :] - PF[..., :-1, :, :]
+    m_expl = b_int * PA[..., -1:, :, :] - PA[..., :-1, :, :]
+
+    def vadv(m, field):
+      return hybrid_vertical_advection(m, field, dp)
+
+    iva = self.include_vertical_advection
+    if self.humidity_key is not None:
+      q = trn[self.humidity_key]
+      adj = 1 + (ps_.R_vapor / ps_.R - 1) * q
+      for key in self.cloud_keys or ():
+        adj = adj - trn[key]
+    else:
+      adj = 1.0
+    weight = lr * bh[:-1] + alpha * db
+    dpf = torch.maximum(dp, 1e-6 * ps)
+    coef = ps_.R * (tn + tref) / dpf * weight * ps * adj - pref
+    total_vort = zn + cor
+    cu = -vn * total_vort * sec2 + ((-vadv(m_full, un) if iva else 0.0) + coef * gxn) * sec2
+    cv = un * total_vort * sec2 + ((-vadv(m_full, vn) if iva else 0.0) + coef * gyn) * sec2
+    tv = (tn + tref) * adj
+    w = tv * lr
+    geo_full = ps_.R * (torch.flip(torch.cumsum(torch.flip(w, (-3,)), -3), (-3,)) - w
+                        + alpha * tv)
+    lt = lref * tn
+    geo_lin = ps_.R * (torch.flip(torch.cumsum(torch.flip(lt, (-3,)), -3), (-3,)) - lt
+                       + aref * tn)
+    ke = (un * un + vn * vn) * sec2 / 2 + (geo_full - geo_lin)
+    vgl = udg * ps * weight / dpf
+
+    def t_omega(tfield, prefix):
+      af = aref * prefix
+      af_above = torch.nn.functional.pad(af, (0, 0, 0, 0, 1, 0))[..., :-1, :, :]
+      return tfield * (vgl - (af + af_above) / dp)
+
+    if self.humidity_key is None:
+      tvar = tn
+    else:
+      gcr, hcr = ps_.R_vapor / ps_.R, ps_.Cp_vapor / ps_.Cp
+      den = 1 + (hcr - 1) * q
+      tvar = tn * ((1 + (gcr - 1) * q) / den) + tref * (((gcr - hcr) * q) / den)
+    temp = tn * dn + ps_.kappa * (t_omega(tref, PA) + t_omega(tvar, PF))
+    if iva:
+      temp = temp + vadv(m_full, tn)
+    if np.unique(self.reference_temperature.ravel()).size > 1:
+      temp = temp + vadv(m_expl, tref.expand_as(tn))
+    lsp = -((dn * da).sum(dim=-3, keepdim=True) * (1 / ps - 1 / self.p_s_ref)
+            + (udg * db).sum(dim=-3, keepdim=True))
+    outs = [cu, cv, ke, temp, un * tn * sec2, vn * tn * sec2, lsp]
+    for k, s in trn.items():
+      trn_t = s * dn + (vadv(m_full, s) if iva else 0.0)
+      outs += [trn_t, un * s * sec2, vn * s * sec2]
+    return outs

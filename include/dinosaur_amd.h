@@ -419,6 +419,54 @@ int dino_modal_epilogue_adjoint(const dino_modal* m, const void* tendency_cot,
                                 const void* unused, void* in_cot,
                                 int64_t members, int dtype, void* stream);
 
+/* dino_modal_create with flags.  DINO_MODAL_HYBRID: the state is on hybrid
+ * sigma-pressure levels (PrimitiveEquationsHybrid,
+ * dinosaur/primitive_equations.py:1960-2362).  dino_modal_prep then writes
+ * the input groups of dino_nodal_hybrid_explicit (the dry sigma groups plus
+ * ln ps itself, whatever humidity_index is: the hybrid humidity terms enter
+ * only through the virtual temperature, :2491-2518), and dino_modal_epilogue
+ * reads its output groups, which are those of the dry sigma kernel. */
+enum { DINO_MODAL_HYBRID = 1 };
+int dino_modal_create_ex(int R, int L, int K, int n_tracers, int humidity_index,
+                         double radius, double gravity, const double* inv_lap,
+                         const double* lap, const double* clip_mask,
+                         const double* up, const double* dn, const double* up2,
+                         const double* dn2, int flags, dino_modal** out);
+
+/* Fused nodal nonlinear terms of PrimitiveEquationsHybrid.explicit_terms.
+ * Replaces compute_diagnostic_state_hybrid, the Simmons-Burridge
+ * coefficients, get_geopotential_diff_hybrid, _get_pgf_lps_coefficient,
+ * hybrid_vertical_advection and the nodal parts of the hybrid
+ * curl_and_div_tendencies, nodal_temperature_vertical_tendency,
+ * nodal_temperature_adiabatic_tendency and nodal_log_pressure_tendency
+ * (dinosaur/primitive_equations.py:1483-1674, 1826-1957, 2032-2362).
+ *
+ * Input groups, each (members, levels): Z D T TRACER[0..n) U V GX GY LNPS
+ * (levels K except GX, GY and the nodal ln ps: 1).  Output groups: CU CV KE
+ * TEMP UT VT LSP (TRN TU TV)[0..n), as dino_nodal_explicit for a dry state;
+ * KE holds the kinetic energy plus the geopotential of the full virtual
+ * temperature at the column's surface pressure minus its linearisation
+ * about the reference state (both take the same -laplacian).
+ * a_boundaries, b_boundaries: K+1 interface coefficients, top first, with a
+ * in the units of exp(ln ps) (nondimensional pressure); ps_ref: reference
+ * surface pressure in the same units; tref: K reference temperatures. */
+typedef struct dino_nodal_hybrid dino_nodal_hybrid;
+int dino_nodal_hybrid_create(int K, int I, int J, int n_tracers,
+                             int humidity_index, const int* cloud_indices,
+                             int n_clouds, int include_vertical_advection,
+                             const double* sec2, const double* coriolis,
+                             const double* a_boundaries,
+                             const double* b_boundaries, const double* tref,
+                             double ps_ref, double r_gas, double kappa,
+                             double r_vapor, double cp, double cp_vapor,
+                             dino_nodal_hybrid** out);
+int dino_nodal_hybrid_destroy(dino_nodal_hybrid* n);
+int64_t dino_nodal_hybrid_fields(const dino_nodal_hybrid* n, int64_t members,
+                                 int output);
+int dino_nodal_hybrid_explicit(const dino_nodal_hybrid* n,
+                               const void* nodal_in, void* nodal_out,
+                               int64_t members, int dtype, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

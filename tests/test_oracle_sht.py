@@ -164,3 +164,28 @@ def test_clip_wavenumbers_kat():
   assert np.all(y[:, -2:] == 0) and np.all(y[:, :-2] == 1)
   with pytest.raises(ValueError):
     g.clip_wavenumbers(x, n=0)
+
+
+def test_vor_div_uv_closed_forms_and_round_trip():
+  """Solid-body rotation u = U cos(lat), v = 0 has vorticity 2 U sin(lat) / a
+  and zero divergence; a pure divergent flow u = 0, v = V cos(lat) sin(lat)
+  has divergence V (1 - 3 sin^2) / a and no vorticity (spherical_harmonic.py:
+  1181-1210).  Band-limited u, v round-trip through vorticity and divergence."""
+  g = sht.Grid.T42(radius=2.0)
+  lon, lat = np.meshgrid(*g.nodal_axes, indexing='ij')
+  cos = np.sqrt(1 - lat ** 2)
+  U = 3.0
+  vort, div = sht.uv_nodal_to_vor_div_modal(g, U * cos, np.zeros_like(cos))
+  np.testing.assert_allclose(g.to_nodal(vort), 2 * U * lat / g.radius, atol=1e-12)
+  assert np.abs(div).max() < 1e-12
+  V = 0.7
+  vort, div = sht.uv_nodal_to_vor_div_modal(g, np.zeros_like(cos), V * cos * lat)
+  np.testing.assert_allclose(g.to_nodal(div), V * (1 - 3 * lat ** 2) / g.radius, atol=1e-12)
+  assert np.abs(vort).max() < 1e-12
+  u, v = sht.vor_div_to_uv_nodal(g, *sht.uv_nodal_to_vor_div_modal(g, U * cos,
+                                                                   V * cos * lat))
+  np.testing.assert_allclose(u, U * cos, atol=1e-12)
+  np.testing.assert_allclose(v, V * cos * lat, atol=1e-12)
+  # a planted sign error in the curl fails the closed form
+  bad = -g.to_nodal(sht.uv_nodal_to_vor_div_modal(g, U * cos, np.zeros_like(cos))[0])
+  assert np.abs(bad - 2 * U * lat / g.radius).max() > 1.0

@@ -50,6 +50,7 @@ SIGNATURES = {
     'dino_sht_workspace_bytes': (ctypes.c_size_t, [_c_p, _c_i64, _c_i]),
     'dino_sht_inverse': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
     'dino_sht_forward': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
+    'dino_sht_inverse_scratch': (_c_i, [_c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
     'dino_sht_inverse_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
     'dino_sht_forward_adjoint': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
     'dino_dft_forward': (_c_i, [_c_p, _c_p, _c_p, _c_i64, _c_i, _c_i, _c_p]),
@@ -103,7 +104,12 @@ SIGNATURES = {
     'dino_nodal_hybrid_destroy': (_c_i, [_c_p]),
     'dino_nodal_hybrid_fields': (_c_i64, [_c_p, _c_i64, _c_i]),
     'dino_nodal_hybrid_explicit': (_c_i, [_c_p, _c_p, _c_p, _c_i64, _c_i, _c_p]),
+    'dino_vregrid_create': (_c_i, [_c_i, _c_i, _c_dp, _c_dp, _c_i, _c_dp, _c_dp,
+                                   ctypes.POINTER(_c_p)]),
+    'dino_vregrid_destroy': (_c_i, [_c_p]),
+    'dino_vregrid_apply': (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_i64, _c_i64, _c_i, _c_p]),
 }
+VREGRID_CONSERVATIVE_SIGMA, VREGRID_CONSERVATIVE_PRESSURE, VREGRID_LINEAR_SIGMA = 0, 1, 2
 MODAL_HYBRID = 1
 
 _lib = None

@@ -131,6 +131,20 @@ int dino_sht_inverse(const dino_plan* plan, const void* modal, void* nodal,
                      void* work, int64_t batch, int dtype, int precision,
                      void* stream);
 
+/* dino_sht_inverse without the workspace argument: the (batch, 2M-1, J)
+ * Fourier rows live in a static device array of DINO_SHT_SCRATCH_BYTES
+ * (64 MiB) that the library holds on every GPU it runs on until the process
+ * ends; the call allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Returns
+ * DINO_EINVAL when dino_sht_workspace_bytes(plan, batch, dtype) exceeds the
+ * array.  Every call on one GPU shares that GPU's array: calls that overlap
+ * in time on one GPU (other streams or threads) race. */
+#define DINO_SHT_SCRATCH_BYTES ((size_t)64 << 20)
+int dino_sht_inverse_scratch(const dino_plan* plan, const void* modal,
+                             void* nodal, int64_t batch, int dtype,
+                             int precision, void* stream);
+
 /* nodal -> modal with quadrature weights.  Replaces
  * RealSphericalHarmonics.transform (:271-280) via Grid.to_modal (:1025-1031). */
 int dino_sht_forward(const dino_plan* plan, const void* nodal, void* modal,
@@ -466,6 +480,40 @@ int64_t dino_nodal_hybrid_fields(const dino_nodal_hybrid* n, int64_t members,
 int dino_nodal_hybrid_explicit(const dino_nodal_hybrid* n,
                                const void* nodal_in, void* nodal_out,
                                int64_t members, int dtype, void* stream);
+
+/* Vertical regridding of nodal columns from hybrid levels (source: K_s
+ * layers with interface coefficients a, b, p = a + b ps) to target levels.
+ * Modes:
+ *   DINO_VREGRID_CONSERVATIVE_SIGMA: conservative (overlap-weighted means)
+ *     in sigma = a / ps + b; target_a = 0, target_b = sigma boundaries.
+ *     Replaces vertical_interpolation.regrid_hybrid_to_sigma
+ *     (dinosaur/vertical_interpolation.py:366-401).
+ *   DINO_VREGRID_CONSERVATIVE_PRESSURE: conservative in pressure a + b ps
+ *     (target: hybrid coefficients).  Replaces regrid_hybrid_to_hybrid
+ *     (:404-440).
+ *   DINO_VREGRID_LINEAR_SIGMA: linear interpolation between layer centres in
+ *     sigma, one linearly extrapolated cell at each end, NaN beyond.
+ *     Replaces interp_hybrid_to_sigma (:303-326, :96-102).
+ * Target cells with no overlap are zero.  a and ps in the same units.
+ * Source boundaries must be non-decreasing down each column.
+ * dino_vregrid_apply: fields (n_fields, K_s, columns) -> out (n_fields, K_t,
+ * columns); surface_pressure (n_surface, columns), field f using slice
+ * f / (n_fields / n_surface). */
+enum {
+  DINO_VREGRID_CONSERVATIVE_SIGMA = 0,
+  DINO_VREGRID_CONSERVATIVE_PRESSURE = 1,
+  DINO_VREGRID_LINEAR_SIGMA = 2,
+};
+typedef struct dino_vregrid dino_vregrid;
+int dino_vregrid_create(int mode, int source_levels, const double* source_a,
+                        const double* source_b, int target_levels,
+                        const double* target_a, const double* target_b,
+                        dino_vregrid** out);
+int dino_vregrid_destroy(dino_vregrid* r);
+int dino_vregrid_apply(const dino_vregrid* r, const void* fields,
+                       const void* surface_pressure, void* out,
+                       int64_t n_fields, int64_t n_surface, int64_t columns,
+                       int dtype, void* stream);
 
 #ifdef __cplusplus
 }

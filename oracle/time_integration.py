@@ -171,3 +171,86 @@ def trajectory_from_step(step_fn, outer_steps, inner_steps, start_with_input=Fal
     return x, frames
 
   return multistep
+
+
+# ---------------------------------------------------------------- digital filter initialisation
+
+
+def _negate(x):
+  if isinstance(x, State):
+    return x.map(lambda v: None if v is None else -v)
+  return -x
+
+
+class TimeReversed:
+  """dx/dt = -F(x) - G(x); the inverse takes -eta (time_integration.py:
+  118-140)."""
+
+  def __init__(self, forward_eq):
+    self.forward_eq = forward_eq
+
+  def explicit_terms(self, x):
+    return _negate(self.forward_eq.explicit_terms(x))
+
+  def implicit_terms(self, x):
+    return _negate(self.forward_eq.implicit_terms(x))
+
+  def implicit_inverse(self, x, eta):
+    return self.forward_eq.implicit_inverse(x, -eta)
+
+
+def accumulate_repeated(step_fn, weights, state):
+  """sum_n w_n step^n(state) for n = 1..N (time_integration.py:645-663)."""
+  acc = None
+  for w in weights:
+    state = step_fn(state)
+    term = state * w if isinstance(state, State) else w * state
+    acc = term if acc is None else acc + term
+  return acc
+
+
+def dfi_lanczos_weights(time_span, cutoff_period, dt):
+  """Lanczos-windowed sinc weights for n = 1..N, N = round(span / (2 dt))
+  (time_integration.py:666-674)."""
+  n_steps = round(time_span / (2 * dt))
+  n = np.arange(1, n_steps + 1)
+  return np.sinc(n / (n_steps + 1)) * np.sinc(n * time_span / (cutoff_period * n_steps))
+
+
+def digital_filter_initialization(equation, ode_solver, filters, time_span, cutoff_period, dt):
+  """Weighted mean of the states N steps forward and N steps backward in
+  time, weights normalised with 1 for the initial state (Lynch & Huang 1992;
+  time_integration.py:677-720)."""
+
+  def f(state):
+    fwd = step_with_filters(ode_solver(equation, dt), filters)
+    bwd = step_with_filters(ode_solver(TimeReversed(equation), dt), filters)
+    w = dfi_lanczos_weights(time_span, cutoff_period, dt)
+    total = 1.0 + 2 * w.sum()
+    w0, w = 1.0 / total, w / total
+    out = state * w0 if isinstance(state, State) else w0 * state
+    return out + accumulate_repeated(fwd, w, state) + accumulate_repeated(bwd, w, state)
+
+  return f
+
+
+def horizontal_diffusion_step_filter(grid, dt, tau, order=1):
+  """exp(-scale (l (l + 1) / a^2)^order) per degree with scale = dt /
+  (tau |lambda_max|^order), applied after each step (time_integration.py:
+  459-486, filtering.py:92-100)."""
+  ev = grid.laplacian_eigenvalues
+  scale = dt / (tau * abs(ev[-1]) ** order)
+  scaling = np.exp(-scale * (-ev) ** order)
+
+  def rescale(x):
+    if x is None:
+      return x
+    shp = np.shape(x)
+    if shp == np.broadcast_shapes(shp, scaling.shape):
+      return (x * scaling).astype(x.dtype)
+    return x
+
+  def apply(u, u_next):
+    return u_next.map(rescale) if isinstance(u_next, State) else rescale(u_next)
+
+  return apply

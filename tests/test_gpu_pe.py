@@ -649,6 +649,33 @@ def test_digital_filter_initialization_damps_gravity_waves(gpu_device):
   assert after < before / 10
 
 
+def test_digital_filter_initialization_matches_oracle(gpu_device):
+  """Digital filter initialization (6 h span, 20 min steps, with a
+  horizontal-diffusion step filter) of the imbalanced T21 L12 state against
+  the oracle's (time_integration.py:118-140, 459-486, 645-720), FP64."""
+  print('[test] digital filter initialization vs oracle', flush=True)
+  eq, st, oeq, ost, coords, ocoords = _setup('T21', 12, False, torch.float64, gpu_device)
+  grid = coords.horizontal
+  _, l = grid.modal_mesh
+  rs = np.random.RandomState(3)
+  sel = grid.mask & (l >= 1) & (l <= 6)
+  dd = rs.normal(size=st.divergence.shape) * sel * 1e-4
+  x0 = pe.State(st.vorticity, st.divergence + torch.as_tensor(dd, device=gpu_device),
+                st.temperature_variation, st.log_surface_pressure)
+  ox0 = ope.State(ost.vorticity, ost.divergence + dd, ost.temperature_variation,
+                  ost.log_surface_pressure)
+  ps = ounits.SimUnits.from_si()
+  dt = ps.nondim(1200.0, time=1)
+  span = ps.nondim(6 * 3600.0, time=1)
+  tau = ps.nondim(8 * 3600.0, time=1)
+  dfi = ti.digital_filter_initialization(
+      eq, ti.imex_rk_sil3, [ti.horizontal_diffusion_step_filter(grid, dt, tau)], span, span, dt)
+  odfi = oti.digital_filter_initialization(
+      oeq, oti.imex_rk_sil3, [oti.horizontal_diffusion_step_filter(ocoords.horizontal, dt, tau)],
+      span, span, dt)
+  _cmp(dfi(x0), odfi(ox0), 1e-10, 'dfi')
+
+
 def test_entries_below_triangle_are_structurally_zero(gpu_device):
   """Values planted at l < |m| (outside the triangular truncation) never reach
   the physical entries -- in the reference either: the transforms, derivative

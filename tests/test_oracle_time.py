@@ -65,3 +65,60 @@ def test_exponential_filter_kat():
   # step filter at dt = 20 min equals attenuation 16
   step = ti.exponential_step_filter(g, 0.175008)
   np.testing.assert_allclose(step(None, x)[0], np.exp(-16 * (np.arange(129) / 128) ** 36), rtol=1e-6)
+
+
+class _Oscillator:
+  """x0 grows linearly, (x1, x2) is a harmonic oscillator of period 2 pi."""
+
+  def explicit_terms(self, x):
+    return np.array([0.1, x[2], -x[1]])
+
+  def implicit_terms(self, x):
+    return np.zeros_like(x)
+
+  def implicit_inverse(self, x, eta):
+    return x
+
+
+def test_dfi_known_answers():
+  """The reference's DFI known answers (time_integration_test.py:305-335)."""
+  acc = ti.accumulate_repeated(lambda x: 2 * x, np.arange(4), 1)
+  assert acc == 0 * 2 + 1 * 4 + 2 * 8 + 3 * 16
+  w = ti.dfi_lanczos_weights(10, 10, 0.1)
+  assert w.size == 50 and (np.diff(w) < 0).all()
+  assert abs(w[0] - 1.0) < 5e-3 and abs(w[-1]) < 5e-3
+  dfi = ti.digital_filter_initialization(_Oscillator(), ti.imex_rk_sil3, [], 20.0, 20.0, 0.01)
+  np.testing.assert_allclose(dfi(np.array([1.0, 1.0, 1.0])), [1.0, 0.0, 0.0], atol=1e-3)
+
+
+def test_dfi_filter_response_closed_form():
+  """On the oscillator the DFI output is the filter's discrete response
+  H(w) = w0 + 2 sum_n w_n cos(n w dt) times the input (the time-symmetric
+  weights cancel the sine parts), to the SIL3 phase error; the linear mode
+  is unchanged.  A planted sign error in the reversed equation fails."""
+  dt, span, cutoff = 0.05, 6.0, 6.0
+  dfi = ti.digital_filter_initialization(_Oscillator(), ti.imex_rk_sil3, [], span, cutoff, dt)
+  out = dfi(np.array([2.0, 1.0, 0.0]))
+  w = ti.dfi_lanczos_weights(span, cutoff, dt)
+  total = 1 + 2 * w.sum()
+  n = np.arange(1, w.size + 1)
+  h = (1 + 2 * (w * np.cos(n * dt)).sum()) / total
+  np.testing.assert_allclose(out, [2.0, h, 0.0], atol=5e-6)
+  orig = ti.TimeReversed.explicit_terms
+  try:
+    ti.TimeReversed.explicit_terms = lambda self, x: self.forward_eq.explicit_terms(x)
+    bad = dfi(np.array([2.0, 1.0, 0.0]))
+  finally:
+    ti.TimeReversed.explicit_terms = orig
+  assert abs(bad[0] - 2.0) > 1e-2 or abs(bad[1] - h) > 1e-3
+
+
+def test_horizontal_diffusion_filter_closed_form():
+  """The top degree decays by exp(-dt / tau) per step, degree 0 is kept."""
+  from oracle import sht
+  g = sht.Grid.T21()
+  f = ti.horizontal_diffusion_step_filter(g, dt=0.3, tau=2.0, order=2)
+  x = np.ones(g.modal_shape)
+  y = f(None, x)
+  np.testing.assert_allclose(y[:, -1], np.exp(-0.3 / 2.0), rtol=1e-12)
+  np.testing.assert_allclose(y[:, 0], 1.0)

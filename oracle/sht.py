@@ -384,3 +384,104 @@ def vor_div_to_uv_nodal(grid, vorticity, divergence, clip=True):
   """spherical_harmonic.py:1198-1210."""
   u, v = get_cos_lat_vector(vorticity, divergence, grid, clip=clip)
   return grid.to_nodal(u) / grid.cos_lat, grid.to_nodal(v) / grid.cos_lat
+
+
+# ---------------------------------------------------------------- fast layout
+
+
+def real_fourier_basis_with_zero_imag(wavenumbers: int, nodes: int) -> np.ndarray:
+  """(I, 2M) real basis with a zero column for the m = 0 sine
+  (fourier.py:91-109): f[:, 0] = 1/sqrt(2 pi), f[:, 1] = 0, then cos(m lon)
+  and sin(m lon) / sqrt(pi) alternating."""
+  if nodes < wavenumbers:
+    raise ValueError('`real_basis` requires nodes >= wavenumbers')
+  lon = 2 * np.pi * np.arange(nodes) / nodes
+  m = np.arange(1, wavenumbers)
+  f = np.empty((nodes, 2 * wavenumbers))
+  f[:, 0] = 1 / np.sqrt(2 * np.pi)
+  f[:, 1] = 0
+  f[:, 2::2] = np.cos(np.outer(lon, m)) / np.sqrt(np.pi)
+  f[:, 3::2] = np.sin(np.outer(lon, m)) / np.sqrt(np.pi)
+  return f
+
+
+def _round_up(x, m):
+  return -(-x // m) * m
+
+
+class FastGrid:
+  """FastSphericalHarmonics (spherical_harmonic.py:411-618) restated with its
+  own einsums: modal (2M, L) rows [m=0, m=0' (always zero), 1 cos, 1 sin,
+  ...], the P table unrepeated (M, J, L) applied to the (sign, m) unstacked
+  rows, optional padding to `base_shape_multiple`."""
+
+  def __init__(self, grid: Grid, base_shape_multiple: int = 1):
+    self.grid = grid
+    self.M, self.L = grid.longitude_wavenumbers, grid.total_wavenumbers
+    self.I, self.J = grid.longitude_nodes, grid.latitude_nodes
+    b = base_shape_multiple
+    self.modal_shape = (_round_up(2 * self.M, 2 * b), _round_up(self.L, b))
+    self.nodal_shape = (_round_up(self.I, b), _round_up(self.J, b))
+    pmx = self.modal_shape[0] - 2 * self.M
+    pmy = self.modal_shape[1] - self.L
+    pnx = self.nodal_shape[0] - self.I
+    pny = self.nodal_shape[1] - self.J
+    f = real_fourier_basis_with_zero_imag(self.M, self.I)
+    self.f = np.pad(f, [(0, pnx), (0, pmx)])
+    x, wp = grid._lat_nodes
+    self.w = np.pad((2 * np.pi / self.I) * wp, [(0, pny)])
+    p = legendre_table(self.M, self.L, x)
+    self.p = np.pad(p, [(0, pmx // 2), (0, pny), (0, pmy)])
+
+  @property
+  def modal_axes(self):
+    m_pos = np.arange(1, self.M)
+    lon_k = np.concatenate([[0, 0], np.stack([m_pos, -m_pos], axis=1).ravel()])
+    lon_k = np.pad(lon_k, [(0, self.modal_shape[0] - 2 * self.M)])
+    return lon_k, np.pad(np.arange(self.L), [(0, self.modal_shape[1] - self.L)])
+
+  @property
+  def mask(self):
+    m, l = np.meshgrid(*self.modal_axes, indexing='ij')
+    i, j = np.meshgrid(*(np.arange(s) for s in self.modal_shape), indexing='ij')
+    return (abs(m) <= l) & (i != 1) & (i < 2 * self.M) & (j < self.L)
+
+  @staticmethod
+  def _unstack(x):
+    return np.reshape(x, x.shape[:-2] + (2, x.shape[-2] // 2) + x.shape[-1:], order='F')
+
+  @staticmethod
+  def _stack(x):
+    return np.reshape(x, x.shape[:-3] + (-1,) + x.shape[-1:], order='F')
+
+  def to_nodal(self, x):
+    x = self._unstack(x)
+    x = np.einsum('mjl,...sml->...smj', self.p, x)
+    x = self._stack(x)
+    return np.einsum('im,...mj->...ij', self.f, x)
+
+  def to_modal(self, z):
+    x = np.einsum('im,...ij->...mj', self.f, self.w * z)
+    x = self._unstack(x)
+    x = np.einsum('mjl,...smj->...sml', self.p, x)
+    return self._stack(x)
+
+  def d_dlon(self, x):
+    """j * (u_down for even rows, -u_up for odd rows), j = row // 2
+    (fourier.py:112-126)."""
+    i = np.arange(x.shape[-2])[:, None]
+    j = i // 2
+    down = np.concatenate([x[..., 1:, :], np.zeros_like(x[..., :1, :])], axis=-2)
+    up = np.concatenate([np.zeros_like(x[..., :1, :]), x[..., :-1, :]], axis=-2)
+    return j * np.where((i + 1) % 2, down, -up)
+
+  def from_real(self, x):
+    """Real-layout (2M-1, L) coefficients in this layout."""
+    out = np.zeros(x.shape[:-2] + self.modal_shape)
+    out[..., 0, :self.L] = x[..., 0, :]
+    out[..., 2:2 * self.M, :self.L] = x[..., 1:, :]
+    return out
+
+  def to_real(self, x):
+    out = np.concatenate([x[..., :1, :self.L], x[..., 2:2 * self.M, :self.L]], axis=-2)
+    return out

@@ -125,3 +125,57 @@ def test_fast_layout_primitive_equations_match_real(gpu_device):
       err = ((x - y).abs().max() / max(y.abs().max().item(), 1e-300)).item()
       print(f'  {what} {fld}: {err:.1e}', flush=True)
       assert err < 1e-11, (what, fld, err)
+
+
+@pytest.mark.parametrize('multiple', [1, 8])
+def test_fast_layout_matches_oracle_fast_restatement(multiple, gpu_device):
+  """Against the oracle's own einsum restatement of the Fast layout (not the
+  Real layout): to_nodal, to_modal, d_dlon and basis metadata (modal axes,
+  mask, shapes), FP64 1e-12 with and without padding; then one SIL3 step of
+  T21 L6 dry primitive equations in the Fast layout against the oracle's
+  Real-layout step relabelled into Fast rows."""
+  from oracle import pe as ope, sht as osht, sigma as osig, time_integration as oti
+  from oracle import units as ounits
+  from dinosaur_amd import workloads
+  print(f'[test] fast layout vs oracle fast restatement, multiple={multiple}', flush=True)
+  og = osht.Grid.T21()
+  ofg = osht.FastGrid(og, base_shape_multiple=multiple)
+  fast = sh.Grid.T21(spherical_harmonics_impl=lambda **kw: sh.FastSphericalHarmonics(
+      **kw, base_shape_multiple=multiple))
+  assert tuple(fast.modal_shape) == ofg.modal_shape
+  assert tuple(fast.nodal_shape) == ofg.nodal_shape
+  np.testing.assert_array_equal(fast.modal_axes[0], ofg.modal_axes[0])
+  np.testing.assert_array_equal(np.asarray(fast.mask), ofg.mask)
+  rs = np.random.RandomState(5)
+  _, l = og.modal_mesh
+  xf = ofg.from_real(rs.normal(size=(4,) + og.modal_shape) * og.mask / (l + 1) ** 1.5)
+  t = torch.as_tensor(xf, device=gpu_device)
+  rel = lambda a, b: np.abs(a - b).max() / np.abs(b).max()
+  zn = fast.to_nodal(t).cpu().numpy()
+  assert rel(zn, ofg.to_nodal(xf)) < 1e-12
+  assert rel(fast.to_modal(torch.as_tensor(zn, device=gpu_device)).cpu().numpy(),
+             ofg.to_modal(zn)) < 1e-12
+  assert rel(fast.d_dlon(t).cpu().numpy(), ofg.d_dlon(xf)) < 1e-14
+  if multiple != 1:
+    return
+  eq, st, coords, ps = workloads.build_primitive('T21', 6, dtype=torch.float64, device=gpu_device)
+  feq, fst, fcoords, _ = workloads.build_primitive(
+      'T21', 6, dtype=torch.float64, device=gpu_device,
+      impl=sh.FastSphericalHarmonics)
+  ocoords = ope.Coords(og, osig.SigmaCoordinates.equidistant(6))
+  oeq = ope.PrimitiveEquationsSigma(np.asarray(eq.reference_temperature, np.float64),
+                                    eq.orography.double().cpu().numpy(), ocoords,
+                                    ounits.SimUnits.from_si())
+  fields = ('vorticity', 'divergence', 'temperature_variation', 'log_surface_pressure')
+  ost = ope.State(*(getattr(st, f).cpu().numpy() for f in fields))
+  dt = ps.nondimensionalize(1200 * units.scales.units.s)
+  ref = oti.imex_rk_sil3(oeq, dt)(ost)
+  fin = pe.State(*(torch.as_tensor(ofg.from_real(getattr(ost, f)), device=gpu_device)
+                   for f in fields))
+  out = ti.imex_rk_sil3(feq, dt)(fin)
+  for f in fields:
+    got = ofg.to_real(getattr(out, f).cpu().numpy())
+    err = rel(got, getattr(ref, f))
+    print(f'  fast SIL3 step {f}: rel err {err:.2e}', flush=True)
+    assert err < 1e-10, (f, err)
+    assert np.abs(getattr(out, f).cpu().numpy()[..., 1, :]).max() == 0

@@ -189,3 +189,31 @@ def test_vor_div_uv_closed_forms_and_round_trip():
   # a planted sign error in the curl fails the closed form
   bad = -g.to_nodal(sht.uv_nodal_to_vor_div_modal(g, U * cos, np.zeros_like(cos))[0])
   assert np.abs(bad - 2 * U * lat / g.radius).max() > 1.0
+
+
+@pytest.mark.parametrize('base', [1, 8])
+def test_fast_layout_restatement_matches_real_layout(base):
+  """The Fast layout's own einsum restatement (zero-imaginary Fourier basis,
+  unrepeated P applied to (sign, m)-unstacked rows) equals the pinned Real
+  layout after the row relabelling (Fast row 0 = Real row 0, Fast row r =
+  Real row r - 1 for r >= 2, row 1 zero), both directions, with and without
+  padding; its longitudinal derivative equals the Real one; the m = 0' row
+  stays zero under to_modal.  A swapped sin/cos stacking order fails."""
+  g = sht.Grid.T21()
+  fg = sht.FastGrid(g, base_shape_multiple=base)
+  rs = np.random.RandomState(0)
+  _, l = g.modal_mesh
+  x = rs.normal(size=(3,) + g.modal_shape) * g.mask / (l + 1)
+  xf = fg.from_real(x)
+  assert np.abs(xf[..., ~fg.mask]).max() == 0
+  z = g.to_nodal(x)
+  zf = fg.to_nodal(xf)
+  np.testing.assert_allclose(zf[..., :g.nodal_shape[0], :g.nodal_shape[1]], z, atol=1e-13)
+  back = fg.to_modal(zf)
+  np.testing.assert_allclose(fg.to_real(back), g.to_modal(z), atol=1e-13)
+  assert np.abs(back[..., 1, :]).max() < 1e-15
+  np.testing.assert_allclose(fg.to_real(fg.d_dlon(xf)), g.d_dlon(x), atol=1e-14)
+  bad = sht.FastGrid(g, base)
+  bad.f = bad.f.copy()
+  bad.f[:, 2::2], bad.f[:, 3::2] = fg.f[:, 3::2], fg.f[:, 2::2]
+  assert np.abs(bad.to_nodal(xf) - zf).max() > 1e-3

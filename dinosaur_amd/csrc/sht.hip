@@ -3,6 +3,14 @@
 // caller-provided workspace holding the (batch, 2M-1, J) Fourier rows.
 #include "dino_common.h"
 
+#include <mutex>
+
+// Fourier rows of dino_sht_inverse_scratch.  Every GPU holds its own copy of
+// this array until the process ends.  In host code the name is a stand-in,
+// not a device address: host code uses only the per-device address that
+// scratch_address() obtains with hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[DINO_SHT_SCRATCH_BYTES];
+
 namespace dino {
 int dft_forward_impl(const dino_plan*, const void*, void*, int64_t, int, int, hipStream_t);
 int dft_inverse_impl(const dino_plan*, const void*, void*, int64_t, int, int, hipStream_t);
@@ -52,6 +60,29 @@ bool take_triangle_flag(int* precision) {
   return t;
 }
 
+// Device address of `scratch` on the current device, looked up once per
+// device (each device has its own copy at its own address).
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices];
+
+int scratch_address(void** out) {
+  int dev = -1;
+  DINO_HIP(hipGetDevice(&dev));
+  DINO_REQUIRE(dev >= 0 && dev < kScratchDevices, DINO_EINVAL,
+               "device %d is beyond the %d devices the scratch table holds", dev,
+               kScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    DINO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    DINO_REQUIRE(p != nullptr, DINO_EHIP, "hipGetSymbolAddress(scratch) gave NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return DINO_OK;
+}
+
 }  // namespace
 
 #define CHECK_OR_RETURN(expr)    \
@@ -64,6 +95,25 @@ extern "C" int dino_sht_inverse(const dino_plan* p, const void* modal, void* nod
                                 int64_t batch, int dtype, int precision, void* stream) {
   CHECK_OR_RETURN(check_common(p, modal, nodal, batch, dtype, precision));
   DINO_REQUIRE(batch == 0 || work, DINO_EINVAL, "NULL workspace");
+  hipStream_t s = as_stream(stream);
+  CHECK_OR_RETURN(leg_inverse(p, modal, work, batch, dtype, precision, s));
+  return dft_inverse_impl(p, work, nodal, batch, dtype, 0, s);
+}
+
+// The same kernels as dino_sht_inverse, with the Fourier rows in `scratch`.
+// The Legendre stage writes every (field, row, node) entry of the rows before
+// the DFT stage reads them, so nothing needs clearing between calls.
+extern "C" int dino_sht_inverse_scratch(const dino_plan* p, const void* modal, void* nodal,
+                                        int64_t batch, int dtype, int precision, void* stream) {
+  CHECK_OR_RETURN(check_common(p, modal, nodal, batch, dtype, precision));
+  if (batch == 0) return DINO_OK;
+  const size_t per_field = (dtype == DINO_F64 ? 8 : 4) * (size_t)p->R * (size_t)p->J;
+  DINO_REQUIRE((uint64_t)batch <= DINO_SHT_SCRATCH_BYTES / per_field, DINO_EINVAL,
+               "batch %lld needs %llu bytes of Fourier rows; the scratch array holds %llu",
+               (long long)batch, (unsigned long long)per_field * (unsigned long long)batch,
+               (unsigned long long)DINO_SHT_SCRATCH_BYTES);
+  void* work = nullptr;
+  CHECK_OR_RETURN(scratch_address(&work));
   hipStream_t s = as_stream(stream);
   CHECK_OR_RETURN(leg_inverse(p, modal, work, batch, dtype, precision, s));
   return dft_inverse_impl(p, work, nodal, batch, dtype, 0, s);

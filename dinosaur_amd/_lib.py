@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, '_lib', 'libdinosaur_amd.so')
 
 DINO_OK, DINO_EINVAL, DINO_ESHAPE, DINO_EHIP, DINO_ENOMEM = range(5)
 DINO_F32, DINO_F64 = 0, 1
-PREC_NATIVE, PREC_F16X3, PREC_BF16X3 = 0, 1, 2
+PREC_NATIVE, PREC_BF16X3 = 0, 2
 # transform stages and launch schedules (dino_plan_set_schedule)
 STAGE_LEGENDRE_INVERSE, STAGE_LEGENDRE_FORWARD, STAGE_DFT_INVERSE, STAGE_DFT_FORWARD = 0, 1, 2, 3
 SCHED_AUTO, SCHED_ROWS, SCHED_TILED, SCHED_TILED_NARROW = 0, 1, 2, 3
@@ -27,7 +27,7 @@ TARGET_ACCUMULATE, TARGET_IDENTITY, TARGET_RANGED = 1, 2, 4
 GATHER_IDENTITY, GATHER_SCALED, GATHER_IMPLICIT_RANGE = 1, 2, 4
 SHT_TRIANGLE_ONLY = 0x100
 PRECISIONS = {'native': PREC_NATIVE, 'float32': PREC_NATIVE, 'float64': PREC_NATIVE,
-              'f16x3': PREC_F16X3, 'bf16x3': PREC_BF16X3}
+              'bf16x3': PREC_BF16X3}
 
 _c_p = ctypes.c_void_p
 _c_i = ctypes.c_int

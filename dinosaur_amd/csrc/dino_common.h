@@ -55,10 +55,6 @@ struct dino_plan {
   int64_t* m_offset;    // (M) start of row l = m of each m block in the table
   double* p64;          // folded table, FP64
   float* p32;           // FP32 copy
-  uint16_t* p16_hi;     // split FP16 hi/lo
-  uint16_t* p16_lo;
-  uint16_t* pbf_hi;     // split BF16 hi/lo
-  uint16_t* pbf_lo;
   double* tw64;         // (I) complex twiddles exp(-2 pi i k / I) as (cos, sin)
   float* tw32;
   // Forward-transform tables in MFMA fragment order (legendre_rows.hip).

@@ -13,7 +13,7 @@
 //
 // Operands are staged in LDS de-interleaved by parity.  Arithmetic:
 //   FP32  : v_mfma_f32_16x16x4_f32 (exact FP32 products, FP32 accumulation)
-//   split : hi + lo halves of both operands (FP16 or BF16), three
+//   split : hi + lo BF16 halves of both operands, three
 //           v_mfma_f32_16x16x32 products (hi*hi + hi*lo + lo*hi) accumulated
 //           in FP32.
 #include "dino_common.h"
@@ -22,7 +22,6 @@ namespace dino {
 namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kThreads = 256;
@@ -51,7 +50,6 @@ struct PolicyF32 {
   }
 };
 
-template <bool BF>
 struct PolicySplit {
   static constexpr int KSTEP = 32;
   template <int PA, int PB>
@@ -59,35 +57,19 @@ struct PolicySplit {
                                 int bcol, int k0, int lane, floatx4 acc) {
     const int kb = k0 + 8 * (lane / 16);
     const int ra = arow + lane % 16, cb = bcol + lane % 16;
-    if constexpr (BF) {
-      bf16x8 ah, al, bh, bl;
+    bf16x8 ah, al, bh, bl;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        float a = As[kb + i][ra], b = Bs[kb + i][cb];
-        __bf16 h = (__bf16)a, g = (__bf16)b;
-        ah[i] = h;
-        al[i] = (__bf16)(a - (float)h);
-        bh[i] = g;
-        bl[i] = (__bf16)(b - (float)g);
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
-      return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
-    } else {
-      halfx8 ah, al, bh, bl;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        float a = As[kb + i][ra], b = Bs[kb + i][cb];
-        _Float16 h = (_Float16)a, g = (_Float16)b;
-        ah[i] = h;
-        al[i] = (_Float16)(a - (float)h);
-        bh[i] = g;
-        bl[i] = (_Float16)(b - (float)g);
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
-      return __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
+    for (int i = 0; i < 8; ++i) {
+      float a = As[kb + i][ra], b = Bs[kb + i][cb];
+      __bf16 h = (__bf16)a, g = (__bf16)b;
+      ah[i] = h;
+      al[i] = (__bf16)(a - (float)h);
+      bh[i] = g;
+      bl[i] = (__bf16)(b - (float)g);
     }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
   }
 };
 
@@ -383,9 +365,9 @@ int legendre_inverse_mfma(const dino_plan* pl, const void* modal, void* fourier,
   const bool narrow = fam == DINO_SCHED_TILED_NARROW;
   switch (precision) {
     case DINO_PREC_NATIVE: return inv_by_width<PolicyF32, 16>(pl, in, out, B, narrow, s);
-    case DINO_PREC_BF16X3: return inv_by_width<PolicySplit<true>, 32>(pl, in, out, B, narrow, s);
+    case DINO_PREC_BF16X3: return inv_by_width<PolicySplit, 32>(pl, in, out, B, narrow, s);
     default:
-      set_error("split precision %d is not available; use DINO_PREC_BF16X3", precision);
+      set_error("unknown precision %d", precision);
       return DINO_EINVAL;
   }
 }
@@ -405,9 +387,9 @@ int legendre_forward_mfma(const dino_plan* pl, const void* fourier, void* modal,
       // narrow: 16 nodes per reduction chunk (half the LDS, more resident workgroups)
       if (fam == DINO_SCHED_TILED_NARROW) return launch_fwd<PolicyF32, 16>(pl, in, out, B, s);
       return launch_fwd<PolicyF32, 32>(pl, in, out, B, s);
-    case DINO_PREC_BF16X3: return launch_fwd<PolicySplit<true>, 32>(pl, in, out, B, s);
+    case DINO_PREC_BF16X3: return launch_fwd<PolicySplit, 32>(pl, in, out, B, s);
     default:
-      set_error("split precision %d is not available; use DINO_PREC_BF16X3", precision);
+      set_error("unknown precision %d", precision);
       return DINO_EINVAL;
   }
 }

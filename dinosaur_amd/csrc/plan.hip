@@ -1,15 +1,12 @@
 // Plan creation: FP64 Gauss nodes and the folded, normalised P_l^m table are
-// generated on the GPU; FP32 and split FP16/BF16 copies and the longitude
-// twiddles are derived from them on the GPU.
+// generated on the GPU; the FP32 copy and the longitude twiddles are derived
+// from them on the GPU.
 //
 // Table layout ("folded triangular"): for each m, rows l = m .. L-1, each row
 // holds Jh = ceil(J/2) values P_l^m(x_j) for the northern nodes
 // j = J - Jh .. J - 1, padded to Jhp.  The southern values follow from
 // P_l^m(-x) = (-1)^(l+m) P_l^m(x), so the table is a quarter of the
 // reference's dense (2M-1, J, L) array (spherical_harmonic.py:250-259).
-#include <hip/hip_bf16.h>
-#include <hip/hip_fp16.h>
-
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -130,25 +127,11 @@ __global__ void legendre_table_kernel(int M, int L, int J, int Jh, int Jhp,
 }
 #pragma clang fp contract(on)
 
-__global__ void split_tables_kernel(int64_t n, const double* __restrict__ p64,
-                                    float* __restrict__ p32,
-                                    uint16_t* __restrict__ h_hi,
-                                    uint16_t* __restrict__ h_lo,
-                                    uint16_t* __restrict__ b_hi,
-                                    uint16_t* __restrict__ b_lo) {
+__global__ void round_table_kernel(int64_t n, const double* __restrict__ p64,
+                                   float* __restrict__ p32) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double v = p64[i];
-  float f = (float)v;
-  p32[i] = f;
-  _Float16 hh = (_Float16)v;
-  _Float16 hl = (_Float16)(v - (double)hh);
-  h_hi[i] = __builtin_bit_cast(uint16_t, hh);
-  h_lo[i] = __builtin_bit_cast(uint16_t, hl);
-  __hip_bfloat16 bh = __float2bfloat16(f);
-  __hip_bfloat16 bl = __float2bfloat16((float)(v - (double)__bfloat162float(bh)));
-  b_hi[i] = __bfloat16_as_ushort(bh);
-  b_lo[i] = __bfloat16_as_ushort(bl);
+  p32[i] = (float)p64[i];
 }
 
 __global__ void twiddle_kernel(int n, double* tw64, float* tw32) {
@@ -182,7 +165,7 @@ void inverse_fragment_fill(const dino_plan* pl, hipStream_t s);
 static void free_plan(dino_plan* p) {
   if (!p) return;
   void* bufs[] = {p->nodes64, p->weights64, p->qw32, p->qw64, p->m_offset,
-                  p->p64, p->p32, p->p16_hi, p->p16_lo, p->pbf_hi, p->pbf_lo,
+                  p->p64, p->p32,
                   p->tw64, p->tw32, p->ftile_off, p->ffrag32, p->ffragbf,
                   p->itile_off, p->ifrag32, p->ifragbf, p->sink};
   for (void* b : bufs)
@@ -263,10 +246,6 @@ extern "C" int dino_plan_create(int M, int L, int I, int J, const double* nodes,
   PLAN_HIP(hipMalloc(&p->m_offset, sizeof(int64_t) * M));
   PLAN_HIP(hipMalloc(&p->p64, sizeof(double) * acc));
   PLAN_HIP(hipMalloc(&p->p32, sizeof(float) * acc));
-  PLAN_HIP(hipMalloc(&p->p16_hi, sizeof(uint16_t) * acc));
-  PLAN_HIP(hipMalloc(&p->p16_lo, sizeof(uint16_t) * acc));
-  PLAN_HIP(hipMalloc(&p->pbf_hi, sizeof(uint16_t) * acc));
-  PLAN_HIP(hipMalloc(&p->pbf_lo, sizeof(uint16_t) * acc));
   PLAN_HIP(hipMalloc(&p->tw64, sizeof(double) * 2 * I));
   PLAN_HIP(hipMalloc(&p->tw32, sizeof(float) * 2 * I));
   PLAN_HIP(hipMemcpy(p->m_offset, off.data(), sizeof(int64_t) * M,
@@ -285,8 +264,7 @@ extern "C" int dino_plan_create(int M, int L, int I, int J, const double* nodes,
   legendre_table_kernel<<<tgrid, 64>>>(M, L, J, p->Jh, p->Jhp, p->nodes64,
                                         p->m_offset, p->p64);
   PLAN_HIP(hipGetLastError());
-  split_tables_kernel<<<(unsigned)((acc + 255) / 256), 256>>>(
-      acc, p->p64, p->p32, p->p16_hi, p->p16_lo, p->pbf_hi, p->pbf_lo);
+  round_table_kernel<<<(unsigned)((acc + 255) / 256), 256>>>(acc, p->p64, p->p32);
   PLAN_HIP(hipGetLastError());
   twiddle_kernel<<<(I + 255) / 256, 256>>>(I, p->tw64, p->tw32);
   PLAN_HIP(hipGetLastError());

@@ -30,8 +30,7 @@ int check_common(const dino_plan* p, const void* a, const void* b, int64_t batch
   DINO_REQUIRE(batch >= 0, DINO_EINVAL, "negative batch");
   DINO_REQUIRE(batch == 0 || (a && b), DINO_EINVAL, "NULL buffer");
   DINO_REQUIRE(dtype == DINO_F32 || dtype == DINO_F64, DINO_EINVAL, "unknown dtype %d", dtype);
-  DINO_REQUIRE(precision == DINO_PREC_NATIVE || precision == DINO_PREC_F16X3 ||
-                   precision == DINO_PREC_BF16X3,
+  DINO_REQUIRE(precision == DINO_PREC_NATIVE || precision == DINO_PREC_BF16X3,
                DINO_EINVAL, "unknown precision %d", precision);
   DINO_REQUIRE(precision == DINO_PREC_NATIVE || dtype == DINO_F32, DINO_EINVAL,
                "split-precision transforms take FP32 data");

@@ -347,8 +347,8 @@ class RealSphericalHarmonics(SphericalHarmonics):
   """Real layout m = [0, +1, -1, ...] x l (spherical_harmonic.py:161-283).
 
   ``transform_precision`` picks the Legendre arithmetic for FP32 data:
-  'float32' (FP32 FMA), 'f16x3' or 'bf16x3' (split precision on MFMA with
-  FP32 accumulation).  FP64 data always uses FP64 arithmetic.
+  'float32' (FP32 operands on MFMA) or 'bf16x3' (split-BF16 precision on
+  MFMA with FP32 accumulation).  FP64 data always uses FP64 arithmetic.
   """
   transform_precision: str = 'float32'
 

@@ -40,7 +40,7 @@ enum { DINO_F32 = 0, DINO_F64 = 1 };
 /* arithmetic of the Legendre contraction (FP32 data only for the split forms) */
 enum {
   DINO_PREC_NATIVE = 0, /* FP32 FMA for FP32 data, FP64 FMA for FP64 data */
-  DINO_PREC_F16X3 = 1,  /* split-FP16 x3 on MFMA, FP32 accumulation */
+  /* 1 is unused (a split-FP16 form loses small-magnitude fields) */
   DINO_PREC_BF16X3 = 2, /* split-BF16 x3 on MFMA, FP32 accumulation */
 };
 

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void vregrid_kernel(const T* __restrict__ in,
   T* y = out + f * Kt * p.cols + col;
   if (p.mode == DINO_VREGRID_LINEAR_SIGMA) {
     // source centres in sigma, extended by one cell at each end; target
-    // centres from fixed sigma boundaries (ta = 0)
+    // centres as the caller's sigma coordinates hold them (ta)
     auto sc = [&](int k) { return 0.5 * ((sa[k] + sa[k + 1]) / ps + sb[k] + sb[k + 1]); };
     // extended abscissa / ordinate index e in [0, Ks + 1]: e = 0 and
     // e = Ks + 1 are the extrapolated cells
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void vregrid_kernel(const T* __restrict__ in,
     };
     int e = 0;
     for (int t = 0; t < Kt; ++t) {
-      const double xt = 0.5 * (tb[t] + tb[t + 1]);
+      const double xt = ta[t];
       double v;
       if (xt < xe(0) || xt > xe(Ks + 1)) {
         v = NAN;
@@ -132,10 +132,16 @@ extern "C" int dino_vregrid_create(int mode, int source_levels, const double* so
   h.insert(h.end(), source_b, source_b + Ks + 1);
   h.insert(h.end(), target_a, target_a + Kt + 1);
   h.insert(h.end(), target_b, target_b + Kt + 1);
-  for (int k = 0; k < Kt; ++k)
-    DINO_REQUIRE(target_b[k + 1] >= target_b[k] && target_a[k + 1] + target_b[k + 1] >=
-                                                       target_a[k] + target_b[k],
-                 DINO_EINVAL, "target boundaries must increase downwards");
+  // Hybrid targets can be checked only column by column (their order depends
+  // on the surface pressure); sigma targets are fixed.
+  if (mode == DINO_VREGRID_CONSERVATIVE_SIGMA)
+    for (int k = 0; k < Kt; ++k)
+      DINO_REQUIRE(target_b[k + 1] >= target_b[k], DINO_EINVAL,
+                   "target sigma boundaries must increase downwards");
+  if (mode == DINO_VREGRID_LINEAR_SIGMA)
+    for (int k = 0; k + 1 < Kt; ++k)
+      DINO_REQUIRE(target_a[k + 1] >= target_a[k], DINO_EINVAL,
+                   "target sigma centres must increase downwards");
   dino_vregrid* r = new dino_vregrid();
   std::memset(r, 0, sizeof(*r));
   r->mode = mode;

@@ -134,7 +134,8 @@ def interp_hybrid_to_sigma(fields, hybrid_coords: hybrid_coordinates.HybridCoord
   """Linear interpolation from hybrid layer centres to sigma layer centres,
   one extrapolated cell at each end, NaN beyond (vertical_interpolation.py:
   303-326)."""
+  # the target centres as the sigma coordinates hold them (float32 midpoints)
+  centres = np.append(np.asarray(sigma_coords.centers, np.float64), 0.0)
   r = _Regridder(_lib.VREGRID_LINEAR_SIGMA, hybrid_coords.a_boundaries,
-                 hybrid_coords.b_boundaries, np.zeros(sigma_coords.layers + 1),
-                 sigma_coords.boundaries)
+                 hybrid_coords.b_boundaries, centres, sigma_coords.boundaries)
   return _map_nonscalar(lambda x: r(x, surface_pressure), fields)

@@ -492,8 +492,10 @@ int dino_nodal_hybrid_explicit(const dino_nodal_hybrid* n,
  *     (target: hybrid coefficients).  Replaces regrid_hybrid_to_hybrid
  *     (:404-440).
  *   DINO_VREGRID_LINEAR_SIGMA: linear interpolation between layer centres in
- *     sigma, one linearly extrapolated cell at each end, NaN beyond.
- *     Replaces interp_hybrid_to_sigma (:303-326, :96-102).
+ *     sigma, one linearly extrapolated cell at each end, NaN beyond;
+ *     target_a[0 .. K_t - 1] = the target sigma centres (as the sigma
+ *     coordinates hold them: the reference's are float32 midpoints),
+ *     target_b unused.  Replaces interp_hybrid_to_sigma (:303-326, :96-102).
  * Target cells with no overlap are zero.  a and ps in the same units.
  * Source boundaries must be non-decreasing down each column.
  * dino_vregrid_apply: fields (n_fields, K_s, columns) -> out (n_fields, K_t,

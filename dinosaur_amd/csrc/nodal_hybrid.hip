@@ -125,6 +125,33 @@ __device__ __forceinline__ double dlog(double x) { return log(x); }
 __device__ __forceinline__ float dexp(float x) { return expf(x); }
 __device__ __forceinline__ double dexp(double x) { return exp(x); }
 
+// Sum carried with its rounding error (Knuth's TwoSum per addition).  The
+// geopotential needs suffix sums over the levels below k; the kernel walks
+// down the column, so it forms them as column total minus prefix, and with
+// both carried exactly to second order the difference keeps the digits a
+// direct reverse sum would (the total is dominated by the reference
+// temperature, the suffix near the surface is a few terms).
+template <typename T>
+struct TwoSumAcc {
+  T hi = 0, lo = 0;
+  __device__ __forceinline__ void add(T x) {
+    // no multiply-add fusion into these additions: the error terms must be
+    // those of the rounded summands, and the same in both passes
+#pragma clang fp contract(off)
+    const T s = hi + x;
+    const T bp = s - hi;
+    lo += (hi - (s - bp)) + (x - bp);
+    hi = s;
+  }
+};
+
+// a - b of two compensated sums
+template <typename T>
+__device__ __forceinline__ T twosum_diff(const TwoSumAcc<T>& a, const TwoSumAcc<T>& b) {
+#pragma clang fp contract(off)
+  return (a.hi - b.hi) + (a.lo - b.lo);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void nodal_hybrid_kernel(const T* __restrict__ in,
                                                            T* __restrict__ out,
@@ -154,7 +181,8 @@ __global__ __launch_bounds__(256) void nodal_hybrid_kernel(const T* __restrict__
     return a;
   };
   // ---- pass A: column totals
-  T TA = 0, TF = 0, GS = 0, GL = 0, LA = 0, LB = 0;
+  T TA = 0, TF = 0, LA = 0, LB = 0;
+  TwoSumAcc<T> GS, GL;
   T p_up = cst[o.ah] + cst[o.bh] * ps;
   for (int k = 0; k < K; ++k) {
     const T u = X_(g.u, K, k), v = X_(g.v, K, k), d = X_(g.d, K, k), t = X_(g.t, K, k);
@@ -166,15 +194,16 @@ __global__ __launch_bounds__(256) void nodal_hybrid_kernel(const T* __restrict__
     TF += dp * d + adv;
     const T p_lo = cst[o.ah + k + 1] + cst[o.bh + k + 1] * ps;
     const T lr = dlog(fmax(p_lo, floor_p) / fmax(p_up, floor_p));
-    GS += (t + cst[o.tref + k]) * adjust(k) * lr;
-    GL += cst[o.lref + k] * t;
+    GS.add((t + cst[o.tref + k]) * adjust(k) * lr);
+    GL.add(cst[o.lref + k] * t);
     LA += d * da;
     LB += udg * db;
     p_up = p_lo;
   }
   *O_(g.lsp, 1, 0) = -(LA * ((T)1 / ps - (T)p.inv_psref) + LB);
   // ---- pass B
-  T PA = 0, PF = 0, PGS = 0, PGL = 0;  // prefixes through level k
+  T PA = 0, PF = 0;  // prefixes through level k
+  TwoSumAcc<T> PGS, PGL;
   T aPA_m = 0, aPF_m = 0;              // alpha_ref_{k-1} * prefix_{k-1}
   T fu_m = 0, fv_m = 0, ft_m = 0, fx_m = 0, ftr_m[HMAXTR];
 #pragma unroll
@@ -248,11 +277,10 @@ __global__ __launch_bounds__(256) void nodal_hybrid_kernel(const T* __restrict__
     // geopotential of the full virtual temperature minus the reference
     // linearisation R [alpha_ref t + sum_{j > k} lr_ref_j t_j]
     const T tv_lr = (t0 + tref) * adj * lr;
-    PGS += tv_lr;
-    const T lref_t = cst[o.lref + k] * t0;
-    PGL += lref_t;
-    const T geo_full = R * ((GS - PGS) + alpha * (t0 + tref) * adj);
-    const T geo_lin = R * ((GL - PGL) + aref * t0);
+    PGS.add(tv_lr);
+    PGL.add(cst[o.lref + k] * t0);
+    const T geo_full = R * (twosum_diff(GS, PGS) + alpha * (t0 + tref) * adj);
+    const T geo_lin = R * (twosum_diff(GL, PGL) + aref * t0);
     *O_(g.ke, K, k) = (u0 * u0 + v0 * v0) * s / (T)2 + (geo_full - geo_lin);
     *O_(g.te, K, k) = temp;
     *O_(g.ut, K, k) = u0 * t0 * s;

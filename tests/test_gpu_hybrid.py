@@ -2,8 +2,10 @@
 modal prep, FP64 vertical-mode implicit kernels with the hybrid matrices)
 against the FP64 oracle on the same seeded inputs:
 
-* explicit terms, FP64, dry / moist with clouds / level-varying T_ref: 1e-10;
-* explicit terms in FP32 against the FP64 oracle: 2e-5 normwise;
+* explicit terms, FP64, dry / moist with clouds / level-varying T_ref: 1e-10
+  (divergence 2e-9, its rounding floor being ~4e-9);
+* explicit terms in FP32 against the FP64 oracle: 2e-5 normwise (divergence
+  2e-2 and ln ps 5e-4: the rounding floors of ps = exp(ln ps) at FP32);
 * implicit terms and implicit inverse, FP64: 1e-11;
 * a 2-member batch equals the members evaluated one by one;
 * 12 SIL3 steps (fused IMEX step on flat buffers), FP64: 1e-9;
@@ -71,12 +73,20 @@ def _setup(k, moist, tref, dtype, dev, members=0, fused=True, seed=3):
   return oeq, peq, ost, gst
 
 
-def _cmp(got, ref, tol, what, member=None):
+def _cmp(got, ref, tol, what, member=None, div_tol=None, lnps_tol=None):
+  """Per-field normwise comparison.  div_tol / lnps_tol (default tol) for the
+  divergence and ln ps tendencies, whose rounding floors are higher: ps =
+  exp(ln ps) with nondimensional ln ps ~ 45 and the Laplacian of the
+  geopotential amplify round-off (tests/test_oracle_hybrid.py::
+  test_hybrid_divergence_rounding_floor: one FP64 round trip of ln ps moves
+  the divergence by ~4e-9), and the ln ps tendency carries 1/ps - 1/ps_ref."""
   sel = (lambda a: a[member]) if member is not None else (lambda a: a)
+  special = {'divergence': div_tol, 'log_surface_pressure': lnps_tol}
   for f in FIELDS:
     err = _rel(getattr(got, f).cpu().numpy(), sel(getattr(ref, f)))
     print(f'  {what} {f}: rel err {err:.2e}', flush=True)
-    assert err < tol, (what, f, err)
+    t = special[f] if special.get(f) is not None else tol
+    assert err < t, (what, f, err)
   for n in ref.tracers:
     err = _rel(got.tracers[n].cpu().numpy(), sel(ref.tracers[n]))
     print(f'  {what} tracer {n}: rel err {err:.2e}', flush=True)
@@ -88,13 +98,17 @@ def _cmp(got, ref, tol, what, member=None):
 def test_hybrid_explicit_terms_fp64(moist, tref, gpu_device):
   print(f'[test] hybrid explicit fp64 moist={moist} tref_varies={tref is not None}', flush=True)
   oeq, peq, ost, gst = _setup(12, moist, tref, torch.float64, gpu_device)
-  _cmp(peq.explicit_terms(gst), oeq.explicit_terms(ost), 1e-10, 'explicit')
+  _cmp(peq.explicit_terms(gst), oeq.explicit_terms(ost), 1e-10, 'explicit', div_tol=2e-9)
 
 
 def test_hybrid_explicit_terms_fp32(gpu_device):
   print('[test] hybrid explicit fp32 vs fp64 oracle (moist)', flush=True)
   oeq, peq, ost, gst = _setup(12, True, None, torch.float32, gpu_device)
-  _cmp(peq.explicit_terms(gst), oeq.explicit_terms(ost), 2e-5, 'explicit fp32')
+  # The same conditioning at FP32 round-off: the oracle itself, given FP32
+  # state arrays, is off its FP64 result by 3.1e-3 (divergence) and 6.1e-5
+  # (ln ps tendency); the other fields by <= 4.3e-6.
+  _cmp(peq.explicit_terms(gst), oeq.explicit_terms(ost), 2e-5, 'explicit fp32', div_tol=2e-2,
+       lnps_tol=5e-4)
 
 
 def test_hybrid_implicit_terms_and_inverse_fp64(gpu_device):
@@ -119,7 +133,7 @@ def test_hybrid_members_match_single(gpu_device):
       assert torch.allclose(getattr(got, f)[e], getattr(ref, f), rtol=0, atol=1e-14), f
     ost_e = ope.State(*(getattr(ost, f)[e] for f in FIELDS),
                       tracers={n: v[e] for n, v in ost.tracers.items()})
-    _cmp(ref, oeq.explicit_terms(ost_e), 1e-10, f'member {e}')
+    _cmp(ref, oeq.explicit_terms(ost_e), 1e-10, f'member {e}', div_tol=2e-9)
 
 
 def test_hybrid_sil3_rollout_fp64(gpu_device):

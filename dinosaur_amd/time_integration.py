@@ -1060,16 +1060,18 @@ class _CudaMemory:
     self.dev = dev
 
   def free(self) -> int:
-    # memory the driver can still hand out; blocks cached by torch's
-    # allocator are not counted: they are sized for earlier requests and may
-    # not hold a large new one
+    # memory the driver can still hand out
     return torch.cuda.mem_get_info(self.dev)[0]
 
-  def cached(self) -> int:
-    return torch.cuda.memory_reserved(self.dev) - torch.cuda.memory_allocated(self.dev)
-
-  def release_cache(self) -> None:
-    torch.cuda.empty_cache()
+  def reusable(self) -> int:
+    # torch's cached segments with nothing allocated in them: the allocator
+    # serves new requests from them (or returns them to the driver when a
+    # request is larger).  Free space inside segments that still hold live
+    # blocks (inactive splits) is fragmented and not counted.
+    st = torch.cuda.memory_stats(self.dev)
+    return max(0, st.get('reserved_bytes.all.current', 0) -
+               st.get('allocated_bytes.all.current', 0) -
+               st.get('inactive_split_bytes.all.current', 0))
 
   def allocated(self) -> int:
     return torch.cuda.memory_allocated(self.dev)
@@ -1090,7 +1092,7 @@ class _GraphBudget:
   """Which rollout segments record their graphs in the forward pass.
 
   An int policy keeps the first `policy` segments.  'auto' keeps a segment
-  while the device's free memory covers
+  while the memory available for it covers
 
       held + 2 * peak + 4 GiB
 
@@ -1102,10 +1104,13 @@ class _GraphBudget:
   graph is still alive, and allocates cotangents of the same sizes.  Before
   any segment has been measured both are estimated from the input bytes
   (12x per step held, 16x peak).  Measurements take the largest seen so
-  far.  When the driver's free memory falls short while torch's allocator
-  caches unused blocks (left by earlier work in the process), the cache is
-  returned to the driver once per rollout and the test repeated.  Resets
-  the device's peak-memory statistics around kept segments."""
+  far.  Available memory is the driver's free memory plus torch's cached
+  segments that hold no live block (reused as they stand); fragmented free
+  space inside partly used segments is not counted.  The cache is never
+  emptied here: returning it to the driver makes every later step allocate
+  its buffers from the driver again, which costs far more than the
+  recomputation a kept graph saves.  Resets the device's peak-memory
+  statistics around kept segments."""
 
   RESERVE = 4 << 30
 
@@ -1118,7 +1123,6 @@ class _GraphBudget:
     self.held = 12 * steps_per_segment * nbytes
     self.peak = 16 * steps_per_segment * nbytes
     self.measured = False
-    self.released = False
 
   def keep_next(self) -> bool:
     if self.policy != 'auto':
@@ -1126,13 +1130,7 @@ class _GraphBudget:
     if self.mem is None:
       return False
     need = self.held + 2 * self.peak + self.RESERVE
-    if self.mem.free() >= need:
-      return True
-    if not self.released and self.mem.cached() > 0:
-      self.released = True
-      self.mem.release_cache()
-      return self.mem.free() >= need
-    return False
+    return self.mem.free() + self.mem.reusable() >= need
 
   def run_kept(self, fn, leaves):
     mem = self.mem

@@ -104,24 +104,26 @@ def test_recompute_stage_plan():
 
 class _FakeMemory:
   """A device of `capacity` bytes: kept segments leave `held` bytes allocated
-  each and peak `transient` above that while they run; `cache` bytes sit in
-  the allocator's cache, fragmented (not usable for a new large block)."""
+  each and peak `transient` above that while they run.  `frag` bytes sit in
+  the allocator's cache as free space inside partly used segments (never
+  reusable for new blocks), `whole` bytes as cached segments holding no live
+  block, which new allocations reuse before asking the driver."""
 
-  def __init__(self, capacity, held, transient, cache=0):
-    self.capacity, self.held, self.transient, self.cache = capacity, held, transient, cache
+  def __init__(self, capacity, held, transient, frag=0, whole=0):
+    self.capacity, self.held, self.transient = capacity, held, transient
+    self.frag, self.whole = frag, whole
     self.alloc = 0
     self._peak = 0
     self.high_water = 0
 
+  def _reserved(self, alloc):
+    return self.frag + max(alloc, self.whole)
+
   def free(self):
-    return self.capacity - self.alloc - self.cache
+    return self.capacity - self._reserved(self.alloc)
 
-  def cached(self):
-    return self.cache
-
-  def release_cache(self):
-    self.releases = getattr(self, 'releases', 0) + 1
-    self.cache = 0
+  def reusable(self):
+    return max(0, self.whole - self.alloc)
 
   def allocated(self):
     return self.alloc
@@ -133,8 +135,9 @@ class _FakeMemory:
     return self._peak
 
   def run_segment(self):
-    self._peak = max(self._peak, self.alloc + self.held + self.transient)
-    self.high_water = max(self.high_water, self.alloc + self.held + self.transient + self.cache)
+    top = self.alloc + self.held + self.transient
+    self._peak = max(self._peak, top)
+    self.high_water = max(self.high_water, self._reserved(top))
     self.alloc += self.held
 
 
@@ -165,34 +168,30 @@ def test_graph_budget_stops_before_the_device_is_full():
 
 
 def test_graph_budget_ignores_fragmented_cache():
-  """Cached but fragmented blocks are not free memory: with 39 GiB of them
-  (the state the allocator was in when a policy that counted them ran out of
-  memory) the budget keeps segments only while the driver's free memory
-  admits them; the cache is returned to the driver once, when it stands in
-  the way, and the device never overflows."""
+  """Free space inside partly used cached segments is not available memory:
+  with 39 GiB of it (the state the allocator was in when a policy that
+  counted it ran out of memory) the budget keeps fewer segments, none
+  beyond what the rest of the device admits, and the device never
+  overflows."""
   clean = _simulate(_FakeMemory(288 * GiB, 16 * GiB, 6 * GiB), 40)
-  mem = _FakeMemory(288 * GiB, 16 * GiB, 6 * GiB, cache=39 * GiB)
+  mem = _FakeMemory(288 * GiB, 16 * GiB, 6 * GiB, frag=39 * GiB)
   frag = _simulate(mem, 40)
-  assert frag.kept == clean.kept and mem.releases == 1
+  assert frag.kept < clean.kept
   assert mem.high_water <= mem.capacity
-  assert mem.alloc + 2 * 22 * GiB <= mem.capacity
+  assert mem.frag + mem.alloc + 2 * 22 * GiB <= mem.capacity
 
 
-def test_graph_budget_with_cache_that_cannot_be_released():
-  """A cache the allocator cannot return (blocks still split by live
-  tensors) leaves the driver's free memory as it was: fewer segments are
-  kept, and none beyond what that memory admits."""
-
-  class Pinned(_FakeMemory):
-    def release_cache(self):
-      self.releases = getattr(self, 'releases', 0) + 1
-
+def test_graph_budget_counts_wholly_free_cached_segments():
+  """Cached segments with no live block (left by an earlier rollout of the
+  same shapes) are reused as they stand: the budget keeps as many segments
+  as on a clean device, without emptying the cache, and the device never
+  overflows."""
   clean = _simulate(_FakeMemory(288 * GiB, 16 * GiB, 6 * GiB), 40)
-  mem = Pinned(288 * GiB, 16 * GiB, 6 * GiB, cache=39 * GiB)
-  frag = _simulate(mem, 40)
-  assert frag.kept < clean.kept and mem.releases == 1
+  mem = _FakeMemory(288 * GiB, 16 * GiB, 6 * GiB, whole=150 * GiB)
+  warm = _simulate(mem, 40)
+  assert warm.kept == clean.kept
   assert mem.high_water <= mem.capacity
-  assert mem.alloc + mem.cache + 2 * 22 * GiB <= mem.capacity
+  assert not hasattr(ti._CudaMemory, 'release_cache')
 
 
 def test_graph_budget_sizes_segments_by_their_peak():
@@ -223,13 +222,17 @@ def test_graph_budget_first_estimate_and_int_policy():
   assert not ti._GraphBudget('auto', [torch.zeros(4)], 1).keep_next()
 
 
-def test_cuda_memory_free_excludes_cached_blocks(monkeypatch):
-  """Free memory is what the driver reports, without the allocator's cached
-  (reserved but unallocated) blocks."""
+def test_cuda_memory_queries(monkeypatch):
+  """Free memory is what the driver reports; reusable cache is torch's
+  reserved memory minus what is allocated minus the free space inside partly
+  used segments (inactive splits)."""
   monkeypatch.setattr(torch.cuda, 'mem_get_info', lambda dev=None: (5 * GiB, 288 * GiB))
-  monkeypatch.setattr(torch.cuda, 'memory_reserved', lambda dev=None: 60 * GiB)
-  monkeypatch.setattr(torch.cuda, 'memory_allocated', lambda dev=None: 20 * GiB)
-  assert ti._CudaMemory(torch.device('cuda', 0)).free() == 5 * GiB
+  monkeypatch.setattr(torch.cuda, 'memory_stats', lambda dev=None: {
+      'reserved_bytes.all.current': 60 * GiB, 'allocated_bytes.all.current': 20 * GiB,
+      'inactive_split_bytes.all.current': 15 * GiB})
+  mem = ti._CudaMemory(torch.device('cuda', 0))
+  assert mem.free() == 5 * GiB
+  assert mem.reusable() == 25 * GiB
 
 
 def test_checkpointed_rollout_auto_policy_with_device_memory(monkeypatch):

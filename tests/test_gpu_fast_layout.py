@@ -159,9 +159,14 @@ def test_fast_layout_matches_oracle_fast_restatement(multiple, gpu_device):
   if multiple != 1:
     return
   eq, st, coords, ps = workloads.build_primitive('T21', 6, dtype=torch.float64, device=gpu_device)
-  feq, fst, fcoords, _ = workloads.build_primitive(
-      'T21', 6, dtype=torch.float64, device=gpu_device,
-      impl=sh.FastSphericalHarmonics)
+  # The Fast-layout equation takes the Real one's parameters relabelled into
+  # Fast rows (the seeded recipe would draw a different orography in the
+  # (2M, L) shape).
+  fcoords = coordinate_systems.CoordinateSystem(
+      sh.Grid.T21(spherical_harmonics_impl=sh.FastSphericalHarmonics), coords.vertical)
+  foro = torch.as_tensor(ofg.from_real(eq.orography.double().cpu().numpy()), device=gpu_device)
+  feq = pe.PrimitiveEquationsSigma(np.asarray(eq.reference_temperature, np.float64), foro,
+                                   fcoords, ps)
   ocoords = ope.Coords(og, osig.SigmaCoordinates.equidistant(6))
   oeq = ope.PrimitiveEquationsSigma(np.asarray(eq.reference_temperature, np.float64),
                                     eq.orography.double().cpu().numpy(), ocoords,

@@ -214,13 +214,32 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
             make_float4(0.f, 0.f, 0.f, 0.f);
       }
     } else {
-      for (int idx = tid; idx < kRowsPerBlock * lt0; idx += 256) {
-        const int q = idx / lt0, c = idx - q * lt0;
+      // Rows of any alignment (odd L): per row, float4 stores from the first
+      // 16-byte boundary on, and single floats for the up to 3 degrees
+      // before it and the up to 3 after the last whole float4.  Work items
+      // per row: n4max float4 slots, then 4 head and 4 tail slots.
+      const int n4max = lt0 / 4, per = n4max + 8;
+      for (int idx = tid; idx < kRowsPerBlock * per; idx += 256) {
+        const int q = idx / per, c = idx - q * per;
         const int row = row0 + q;
         if (row >= nrows) continue;
         int b, r;
         row_field(m, row, &b, &r);
-        modal[((int64_t)b * Rrows + r) * L + c] = 0.f;
+        float* const p = modal + ((int64_t)b * Rrows + r) * L;
+        const int head = (int)((4 - ((reinterpret_cast<uintptr_t>(p) >> 2) & 3)) & 3);
+        const int h = head < lt0 ? head : lt0;
+        const int n4 = (lt0 - h) / 4;
+        if (c < n4max) {
+          if (c < n4) *reinterpret_cast<float4*>(p + h + 4 * c) = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+          const int j = c - n4max;
+          if (j < 4) {
+            if (j < h) p[j] = 0.f;
+          } else {
+            const int t = h + 4 * n4 + (j - 4);
+            if (t < lt0) p[t] = 0.f;
+          }
+        }
       }
     }
   }

@@ -15,13 +15,15 @@
 //   stage 2: [V; H V; dsigma^T V] z -- the stage-1 accumulators are the B
 //            operand as they stand: f64 accumulator register i of row tile it
 //            holds row 16 it + 4 i + kg, which is row 4 s + kg of k-step
-//            s = 4 it + i.
+//            s = 4 it + i.  Only the k-steps s < KP / 4 carry nonzero rows of
+//            z (rows >= K vanish), so stage 2 reduces over those alone.
 // Matrix operands come from fragment-ordered tables built at create time,
 // copied into LDS once per 8-wave block (one 8-byte LDS read per lane per
 // MFMA, 512 contiguous bytes per wave).  Levels are
 // padded to KP = 4 ceil(K / 4); the H V block starts at row KP, so the
 // temperature fragments a lane loaded for stage 1 are the ones its stage-2
-// output rows need.
+// output rows need, and the dsigma^T V row follows it at row KP + K (not
+// 2 KP), so K = 37 takes 5 stage-2 row tiles instead of 6.
 #include <cstring>
 #include <functional>
 #include <vector>
@@ -39,16 +41,21 @@ __device__ __forceinline__ doublex4 mma64(double a, double b, doublex4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// NS = KP / 4 reduction steps per stage-1 block; T1 = row tiles of stage 1,
-// T2 = row tiles of stage 2 (2 KP + 1 rows).
+// NS = KP / 4 reduction steps per stage-1 block and per stage-2 tile;
+// T1 = row tiles of stage 1; T2MAX = the most row tiles stage 2 takes
+// (KP + K + 1 <= 2 KP + 1 rows; the count for a given K is stage2_tiles).
 template <int NS>
 struct Shape {
   static constexpr int KP = 4 * NS;
   static constexpr int T1 = (KP + 15) / 16;
-  static constexpr int S2 = 4 * T1;  // stage-2 reduction steps (z rows 0 .. 16 T1)
-  static constexpr int T2 = (2 * KP + 1 + 15) / 16;
+  static constexpr int T2MAX = (2 * KP + 1 + 15) / 16;
   static constexpr int TT = (KP + 1 + 15) / 16;  // terms: [H; dsigma] row tiles
 };
+
+__host__ __device__ inline int stage2_tiles(int K) {
+  const int KP = 4 * ((K + 3) / 4);
+  return (KP + K + 1 + 15) / 16;
+}
 
 template <typename T>
 __device__ __forceinline__ double ldd(const T* p) {
@@ -99,13 +106,14 @@ void solve_mfma_kernel(
   using S = Shape<NS>;
   // the block's waves share the operand tables through LDS (one L2 read per
   // block instead of one per wave and MFMA)
-  constexpr int N1 = S::T1 * NS * 64, N2 = S::T2 * S::S2 * 64;
+  constexpr int N1 = S::T1 * NS * 64, N2 = S::T2MAX * NS * 64;
   __shared__ double t1a[N1], t1b[N1], t2[N2];
   for (int i = threadIdx.x; i < N1; i += blockDim.x) {
     t1a[i] = t1a_g[i];
     t1b[i] = t1b_g[i];
   }
-  for (int i = threadIdx.x; i < N2; i += blockDim.x) t2[i] = t2_g[i];
+  const int T2 = stage2_tiles(K);
+  for (int i = threadIdx.x; i < T2 * NS * 64; i += blockDim.x) t2[i] = t2_g[i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -147,43 +155,48 @@ void solve_mfma_kernel(
       cb[it] = mma64(t1b[(it * NS + s) * 64 + lane], xt[s], cb[it]);
     }
   }
-  // z = (C_a - eta lam (C_b + w lnps)) / (1 - eta^2 lam d), rows >= K vanish
-  double z[S::S2];
+  // z = (C_a - eta lam (C_b + w lnps)) / (1 - eta^2 lam d), rows >= K vanish;
+  // stage 2 reads the k-steps s < NS (rows < KP) only
+  double z[NS];
 #pragma unroll
   for (int it = 0; it < S::T1; ++it) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
+      if (4 * it + i >= NS) continue;
       const int row = 16 * it + 4 * i + kg;
       const double c1 = TRANS ? eta : eta * lam_l;
       const double rhs = ca[it][i] - c1 * (cb[it][i] + wvec[row] * lp);
       z[4 * it + i] = rhs / (1.0 - eta * eta * lam_l * dvec[row]);
     }
   }
-  // stage 2, one output row tile at a time
+  // stage 2, one output row tile at a time: rows [0, KP) delta, [KP, KP + K)
+  // temperature, KP + K ln ps
   const double c2 = TRANS ? eta * lam_l : eta;
+  const int lrow = S::KP + K;
   double sacc = 0.0;
 #pragma unroll
-  for (int gt = 0; gt < S::T2; ++gt) {
+  for (int gt = 0; gt < S::T2MAX; ++gt) {
+    if (gt >= T2) continue;  // uniform: K = KP takes the last tile
     doublex4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int s = 0; s < S::S2; ++s) acc = mma64(t2[(gt * S::S2 + s) * 64 + lane], z[s], acc);
+    for (int s = 0; s < NS; ++s) acc = mma64(t2[(gt * NS + s) * 64 + lane], z[s], acc);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int row = 16 * gt + 4 * i + kg;
       if (row < S::KP) {
         if (cok && row < K) div_out[base + (int64_t)row * cols] = (T)acc[i];
-      } else if (row < 2 * S::KP) {
+      } else if (row < lrow) {
         const int q = row - S::KP;  // = 4 s + kg with s = 4 gt + i - NS
         const int s = 4 * gt + i - NS;
-        if (cok && q < K && s >= 0 && s < NS)
+        if (cok && s >= 0 && s < NS)
           temp_out[base + (int64_t)q * cols] = (T)(xt[s] - c2 * acc[i]);
-      } else if (row == 2 * S::KP) {
+      } else if (row == lrow) {
         sacc = acc[i];
       }
     }
   }
-  // the lnps row lives in one k-group; route it to the lanes of kg = 0
-  if (kg == ((2 * S::KP) & 3) && cok) lnps_out[b * cols + cc] = (T)(lp - c2 * sacc);
+  // the lnps row lives in one k-group
+  if (kg == (lrow & 3) && cok) lnps_out[b * cols + cc] = (T)(lp - c2 * sacc);
 }
 
 // TRANS = false: the implicit terms.  TRANS = true: their transpose
@@ -300,7 +313,7 @@ int implicit_mfma_tables(int K, const double* G, const double* H, const double* 
                          const double* dsigma, const double* V, const double* Vinv,
                          const double* d, double r_gas, std::vector<double>* out) {
   const int NS = implicit_mfma_steps(K), KP = 4 * NS;
-  const int T1 = (KP + 15) / 16, S2 = 4 * T1, T2 = (2 * KP + 1 + 15) / 16;
+  const int T1 = (KP + 15) / 16, T2 = stage2_tiles(K);
   const int TT = (KP + 1 + 15) / 16;
   auto at = [&](const double* A, int r, int c) { return A[(size_t)r * K + c]; };
   std::vector<double> WG((size_t)K * K, 0.0), HV((size_t)K * K, 0.0), w(K, 0.0), sV(K, 0.0);
@@ -335,11 +348,11 @@ int implicit_mfma_tables(int K, const double* G, const double* H, const double* 
   // [4] d, [5] tg = G, [6] th = [H; dsigma^T], [7] R T_ref
   fragment_table(out[0], T1, NS, [&](int r, int c) { return modes && in(r, c) ? at(Vinv, r, c) : 0.0; });
   fragment_table(out[1], T1, NS, [&](int r, int c) { return in(r, c) ? WG[(size_t)r * K + c] : 0.0; });
-  fragment_table(out[2], T2, S2, [&](int r, int c) {
+  fragment_table(out[2], T2, NS, [&](int r, int c) {
     if (!modes || c >= K) return 0.0;
     if (r < KP) return r < K ? at(V, r, c) : 0.0;
-    if (r < 2 * KP) return r - KP < K ? HV[(size_t)(r - KP) * K + c] : 0.0;
-    return r == 2 * KP ? sV[c] : 0.0;
+    if (r < KP + K) return HV[(size_t)(r - KP) * K + c];
+    return r == KP + K ? sV[c] : 0.0;
   });
   vec16(out[3], [&](int r) { return w[r]; });
   vec16(out[4], [&](int r) { return d ? d[r] : 0.0; });
@@ -354,11 +367,11 @@ int implicit_mfma_tables(int K, const double* G, const double* H, const double* 
   // [12] H^T, [13] [G^T; (R T_ref)^T], [14] dsigma
   fragment_table(out[8], T1, NS, [&](int r, int c) { return modes && in(r, c) ? at(V, c, r) : 0.0; });
   fragment_table(out[9], T1, NS, [&](int r, int c) { return in(r, c) ? HV[(size_t)c * K + r] : 0.0; });
-  fragment_table(out[10], T2, S2, [&](int r, int c) {
+  fragment_table(out[10], T2, NS, [&](int r, int c) {
     if (!modes || c >= K) return 0.0;
     if (r < KP) return r < K ? at(Vinv, c, r) : 0.0;
-    if (r < 2 * KP) return r - KP < K ? WG[(size_t)c * K + (r - KP)] : 0.0;
-    return r == 2 * KP ? w[c] : 0.0;
+    if (r < KP + K) return WG[(size_t)c * K + (r - KP)];
+    return r == KP + K ? w[c] : 0.0;
   });
   vec16(out[11], [&](int r) { return sV[r]; });
   fragment_table(out[12], T1, NS, [&](int r, int c) { return in(r, c) ? at(H, c, r) : 0.0; });

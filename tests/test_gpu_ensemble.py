@@ -97,3 +97,55 @@ def test_config3_mixed_ensemble_graph_ticks(gpu_device):
     assert torch.equal(a, b)
   days = [s.sim_time for s in graphed.states()]
   assert np.allclose(days, days[0]), days
+
+
+def test_ensemble_members_match_oracle(gpu_device):
+  """One member of each group of a graph-captured mixed ensemble (T21, 3
+  members, 20 min every tick + T42, 2 members, 40 min every other tick;
+  moist L6 with 2 cloud tracers; FP64) against the oracle stepping that
+  member alone from the same initial state with the same SIL3 + filter: 4
+  ticks = 4 T21 steps and 2 T42 steps, every field within 1e-10."""
+  from oracle import pe as ope, sht as osht, sigma as osig, time_integration as oti
+  from oracle import units as ounits
+  print('[test] mixed ensemble members vs oracle (FP64)', flush=True)
+  clouds = ('specific_cloud_liquid_water_content', 'specific_cloud_ice_water_content')
+  specs = (('T21', 3, 0, 1200.0, 1, 4), ('T42', 2, 3, 2400.0, 2, 2))
+  groups, refs = [], []
+  for name, members, first, dt_s, every, steps in specs:
+    eq, st, coords, ps = workloads.build_primitive(name, 6, moist=True, members=members,
+                                                   device=gpu_device, first_member=first,
+                                                   dtype=torch.float64)
+    dt = ps.nondimensionalize(dt_s * units_lib.scales.units.s)
+    step = ti.step_with_filters(ti.imex_rk_sil3(eq, dt),
+                                [ti.exponential_step_filter(coords.horizontal, dt)])
+    flat = st.as_flat()
+    groups.append(ensemble.Group(name, step, flat, every))
+    og = getattr(osht.Grid, name)()
+    oeq = ope.PrimitiveEquationsSigma(
+        np.asarray(eq.reference_temperature, np.float64), eq.orography.double().cpu().numpy(),
+        ope.Coords(og, osig.SigmaCoordinates.equidistant(6)), ounits.SimUnits.from_si(),
+        humidity_key='specific_humidity', cloud_keys=clouds)
+    ostep = oti.step_with_filters(oti.imex_rk_sil3(oeq, dt),
+                                  [oti.exponential_step_filter(og, dt)])
+    m = 1
+    y = ope.State(*(getattr(st, f)[m].cpu().numpy() for f in
+                    ('vorticity', 'divergence', 'temperature_variation',
+                     'log_surface_pressure')),
+                  tracers={k: v[m].cpu().numpy() for k, v in st.tracers.items()})
+    for _ in range(steps):
+      y = ostep(y)
+    refs.append(y)
+  ens = ensemble.MixedTruncationEnsemble(groups, graph=True, concurrent=True)
+  ens.run(4)
+  for (name, *_), got, ref in zip(specs, ens.states(), refs):
+    for f in ('vorticity', 'divergence', 'temperature_variation', 'log_surface_pressure'):
+      a, b = getattr(got, f)[1].cpu().numpy(), getattr(ref, f)
+      err = np.abs(a - b).max() / np.abs(b).max()
+      print(f'  {name} member 1 {f}: {err:.2e}', flush=True)
+      assert err < 1e-10, (name, f, err)
+    for k, b in ref.tracers.items():
+      a = got.tracers[k][1].cpu().numpy()
+      scale = np.abs(b).max()
+      err = np.abs(a - b).max() / scale if scale else np.abs(a).max()
+      print(f'  {name} member 1 {k}: {err:.2e}', flush=True)
+      assert err < 1e-10, (name, k, err)

@@ -698,3 +698,33 @@ def test_entries_below_triangle_are_structurally_zero(gpu_device):
     a = step(st).flat_buffer()
     b = step(pe.State.from_flat(noisy, *st._flat[1:])).flat_buffer()
   assert torch.equal(a * mask, b * mask)
+
+
+@pytest.mark.parametrize('grid_name,layers', [('T21', 12), ('TL127', 8)])
+def test_rest_atmosphere_recipe_matches_oracle(grid_name, layers, gpu_device):
+  """The product's rest-atmosphere recipe (primitive_equations_states.py:
+  37-106: barometric ln ps over orography, zero flow, isothermal reference
+  temperatures) against the oracle's on the same seeded orography, FP64:
+  state fields 1e-12, orography and reference temperatures exactly."""
+  from oracle import states as ostates
+  from dinosaur_amd import primitive_equations_states as pes
+  print(f'[test] rest atmosphere recipe vs oracle {grid_name} L{layers}', flush=True)
+  grid = getattr(sh.Grid, grid_name)()
+  og = getattr(osht.Grid, grid_name)()
+  coords = coordinate_systems.CoordinateSystem(
+      grid, sigma_coordinates.SigmaCoordinates.equidistant(layers))
+  ocoords = ope.Coords(og, osig.SigmaCoordinates.equidistant(layers))
+  oro = workloads.scale_orography(og.to_nodal(workloads.orography_modal_coefficients(og)))
+  state_fn, aux = pes.isothermal_rest_atmosphere(
+      coords, units.SimUnits.from_si(), surface_height=oro, device=gpu_device,
+      dtype=torch.float64)
+  st = state_fn()
+  ost, oro_nodal, tref = ostates.isothermal_rest_atmosphere(
+      ocoords, ounits.SimUnits.from_si(), surface_height_nondim=oro)
+  np.testing.assert_array_equal(aux['orography'], oro_nodal)
+  np.testing.assert_array_equal(aux['reference_temperature'], tref)
+  for f in ('vorticity', 'divergence', 'temperature_variation', 'log_surface_pressure'):
+    a, b = getattr(st, f).cpu().numpy(), getattr(ost, f)
+    err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+    print(f'  {f}: {err:.2e}', flush=True)
+    assert a.shape == b.shape and err < 1e-12, (f, err)

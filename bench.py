@@ -71,8 +71,9 @@ def _parse():
   ap.add_argument('--reps', type=int, default=10)
   ap.add_argument('--profile-step', action='store_true',
                   help='build the workload, tune, run one warm-up and one timed forward + '
-                       'adjoint step (one-step checkpoint, recomputed) and exit: the process '
-                       'to put under rocprofv3 for per-kernel times and counters')
+                       'adjoint step (one-step checkpoint, recomputed; one idle second before '
+                       'and after it) and exit: the process to put under rocprofv3 for '
+                       'per-kernel times and counters')
   ap.add_argument('--workload', default='tl255', choices=['tl255', 'ensemble'],
                   help='tl255: the headline fwd+adjoint line; ensemble: the mixed-truncation '
                        'ensemble (32 T85 + 24 T42 + 8 TL127, moist L24), forward ticks')
@@ -587,9 +588,12 @@ def main():
     rollout_fwd_adj(torch, step, state, coords, targets, 1)
     _sync(torch)
     print('[bench] profile step: warm-up done', file=sys.stderr, flush=True)
+    # an idle second on each side marks the timed step in a kernel trace
+    time.sleep(1.0)
     t0 = time.perf_counter()
     rollout_fwd_adj(torch, step, state, coords, targets, 1)
     _sync(torch)
+    time.sleep(1.0)
     print(f'[bench] profile step: {1e3 * (time.perf_counter() - t0):.1f} ms', file=sys.stderr,
           flush=True)
     return

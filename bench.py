@@ -593,9 +593,9 @@ def main():
     t0 = time.perf_counter()
     rollout_fwd_adj(torch, step, state, coords, targets, 1)
     _sync(torch)
+    elapsed = time.perf_counter() - t0
     time.sleep(1.0)
-    print(f'[bench] profile step: {1e3 * (time.perf_counter() - t0):.1f} ms', file=sys.stderr,
-          flush=True)
+    print(f'[bench] profile step: {1e3 * elapsed:.1f} ms', file=sys.stderr, flush=True)
     return
   # One-step checkpoint segments.  The headline keeps no graph (every step is
   # recomputed in the backward pass): the configured 48 h rollout (576 steps)

@@ -31,7 +31,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kRowsPerBlock = 64;
 constexpr int kTileDeg = 32;
-constexpr int kOutPitch = kTileDeg + 1;  // output staging row pitch (floats)
+constexpr int kRingPitch = 68;  // output ring row pitch (floats): 64 degrees + 4
+static_assert(kRingPitch % 4 == 0 && kRingPitch >= 64, "float4-aligned rows of 64 degrees");
 
 __device__ __forceinline__ void row_field(int m, int q, int* b, int* r) {
   if (m == 0) {
@@ -74,12 +75,58 @@ __device__ __forceinline__ floatx4 ld4(const float4* p) {
   return *reinterpret_cast<const floatx4*>(p);
 }
 
+// Float index mod 32 of an address (its position in a 128-byte line).
+__device__ __forceinline__ int line_pos(const float* p) {
+  return (int)((reinterpret_cast<uintptr_t>(p) >> 2) & 31);
+}
+
+// Zero-block extent of a row of zonal wavenumber m starting at p: the
+// degrees [0, n) below the 128-byte line that holds degree lt0 (the tile loop
+// writes that line, zeros included); n < 0 when that line starts before p.
+__device__ __forceinline__ int zero_extent(const float* p, int m, int L) {
+  const int lt0 = leg_tile_start(m, L);
+  return lt0 - ((line_pos(p) + lt0) & 31);
+}
+
+// Number of floats of row (b, r + 1) that the writer of row (b, r) stores
+// as zeros: the head of row r + 1 up to its first line boundary, when that
+// head lies in row r + 1's zero block (the line shared by the two rows is
+// then written whole, by one writer); 0 otherwise.  next = start of row r + 1.
+__device__ __forceinline__ int next_row_head(const float* next, int r, int Rrows, int L) {
+  if (r + 1 >= Rrows) return 0;
+  const int s = (32 - line_pos(next)) & 31;
+  return zero_extent(next, (r + 2) >> 1, L) >= 0 ? s : 0;
+}
+
+// Zero block of the 16 rows strip0 .. strip0 + 15 (one wave), skipped on
+// request: degrees [s, n) of each row, n = zero_extent, s = the row's first
+// line boundary (the head [0, s) is stored by the previous row's writer, see
+// next_row_head), so only whole 128-byte lines.  One row per step with
+// uniform (scalar) row addresses.
+__device__ __forceinline__ void zero_block(float* __restrict__ modal, int m, int strip0,
+                                           int nrows, int Rrows, int L, int lt0, int lane,
+                                           int skip_zero) {
+  if (lt0 == 0 || skip_zero) return;
+  const int s0 = __builtin_amdgcn_readfirstlane(strip0);
+  const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < 16 && s0 + i < nrows; ++i) {
+    int b, r;
+    row_field(m, s0 + i, &b, &r);
+    float* const p = modal + ((int64_t)b * Rrows + r) * L;
+    const int n = zero_extent(p, m, L);
+    const int s = (32 - line_pos(p)) & 31;
+    for (int c4 = lane; 4 * c4 < n - s; c4 += 64)
+      *reinterpret_cast<floatx4*>(p + s + 4 * c4) = zero;
+  }
+}
+
 // VIN: J % 8 == 0 and a 16-byte aligned Fourier buffer (coalesced float4 row
-// loads staged through LDS), otherwise a scalar gather for any J.
-// VOUT: L % 4 == 0 and a 16-byte aligned modal buffer (float2 / float4
-// stores), otherwise scalar stores for any L (TL grids have odd L).
-template <bool BF, int KS, bool VIN, bool VOUT>
-__global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
+// loads staged through LDS), otherwise a scalar gather for any J.  The modal
+// rows may have any length and alignment (TL grids have odd L).  Up to 128
+// northern nodes (KS <= 4) the registers allow 3 waves per SIMD, which the
+// LDS budget (3 workgroups per CU) matches.
+template <bool BF, int KS, bool VIN>
+__global__ __launch_bounds__(256, KS <= 4 ? 3 : 1) void leg_fwd_rows_kernel(
     const float* __restrict__ fourier, float* __restrict__ modal,
     const float4* __restrict__ ftab, const int64_t* __restrict__ ftile_off,
     float* __restrict__ sink, int B, int L, int J, int Jh, int Rrows, int mpair, int skip_zero) {
@@ -92,7 +139,8 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
   constexpr int STG4 = kRowsPerBlock * PITCH / 4;
   constexpr int LDS4 = (2 * TF4 > STG4) ? 2 * TF4 : STG4;
   __shared__ floatx4 lds[LDS4];
-  __shared__ float ostage[VOUT ? 1 : 4 * 16 * kOutPitch];
+  __shared__ floatx4 ring4[4 * 16 * kRingPitch / 4];  // float4 aligned
+  float* const ring = reinterpret_cast<float*>(ring4);
   const int m = m_of_block(blockIdx.y, gridDim.y, mpair);
   const int nrows = (m == 0) ? B : 2 * B;
   const int row0 = blockIdx.x * kRowsPerBlock;
@@ -116,26 +164,31 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
   float4 x0[KT], x1[KT];
   if constexpr (VIN) {
-    // Coalesced loads: item (row rho of the wave's strip, 4-node chunk c);
-    // consecutive lanes take consecutive chunks of a row, northern half
-    // ascending and its mirror image descending.  Addresses are clamped and
-    // invalid items zeroed by selects, so all loads issue back to back.
+    // Coalesced loads: lane = (row rho of the wave's strip, 4-node chunk c)
+    // with CPR chunks per half row (the whole reduction width) and RPI rows
+    // per instruction, both compile-time, so no index division; northern
+    // half ascending, its mirror image descending.  Chunks past Jh / 4 and
+    // rows past the batch read a clamped in-range address and are zeroed by
+    // selects, so the loads issue back to back without branches.
+    constexpr int CPR = 8 * KS, RPI = 64 / CPR;
+    static_assert(KT * RPI == 16, "KT load instructions cover the wave's 16 rows");
     const int H4 = Jh / 4;
+    const int c = lane % CPR, cc = c < H4 ? c : H4 - 1;
     floatx4 u[KT], d[KT];
 #pragma unroll
     for (int it = 0; it < KT; ++it) {
-      const int idx = lane + 64 * it;
-      const int rho = idx / H4, c = idx - rho * H4;
-      const int row = row0 + wave * 16 + rho;
-      const bool ok = rho < 16 && row < nrows;
-      int b = 0, r = 0;
-      if (ok) row_field(m, row, &b, &r);
+      const int row = row0 + wave * 16 + lane / CPR + RPI * it;
+      const bool ok = c < H4 && row < nrows;
+      int b, r;
+      row_field(m, row < nrows ? row : nrows - 1, &b, &r);
       const float* f = fourier + ((int64_t)b * Rrows + r) * J;
-      const floatx4 uu = *reinterpret_cast<const floatx4*>(f + (ok ? J - Jh + 4 * c : 0));
-      const floatx4 dr = *reinterpret_cast<const floatx4*>(f + (ok ? Jh - 4 - 4 * c : 0));
+      const floatx4 uu = *reinterpret_cast<const floatx4*>(f + J - Jh + 4 * cc);
+      const floatx4 dr = *reinterpret_cast<const floatx4*>(f + Jh - 4 - 4 * cc);
       u[it] = ok ? uu : zero;
       d[it] = ok ? floatx4{dr[3], dr[2], dr[1], dr[0]} : zero;
     }
+    // The zero block is written while these loads are in flight.
+    zero_block(modal, m, row0 + wave * 16, nrows, Rrows, L, lt0, lane, skip_zero);
     // Transpose through LDS into A-operand fragments, X0 first, then X1.
     float* stg = reinterpret_cast<float*>(lds);
 #pragma unroll
@@ -143,10 +196,8 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
       const float sp = pass == 0 ? sgn : -sgn;
 #pragma unroll
       for (int it = 0; it < KT; ++it) {
-        const int idx = lane + 64 * it;
-        const int rho = idx / H4, c = idx - rho * H4;
-        if (rho < 16)
-          *reinterpret_cast<floatx4*>(stg + (wave * 16 + rho) * PITCH + 4 * c) = u[it] + sp * d[it];
+        const int rho = lane / CPR + RPI * it;
+        *reinterpret_cast<floatx4*>(stg + (wave * 16 + rho) * PITCH + 4 * c) = u[it] + sp * d[it];
       }
       wave_lds_sync();  // each wave stages only its own 16 rows
 #pragma unroll
@@ -188,6 +239,7 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
       x1[t] = make_float4(u[0] - sgn * d[0], u[1] - sgn * d[1], u[2] - sgn * d[2],
                           u[3] - sgn * d[3]);
     }
+    zero_block(modal, m, row0 + wave * 16, nrows, Rrows, L, lt0, lane, skip_zero);
   }
   bf16x8 h0[BF ? KS : 1], l0[BF ? KS : 1], h1[BF ? KS : 1], l1[BF ? KS : 1];
   if constexpr (BF) {
@@ -200,70 +252,55 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
     }
   }
 
-  // Zero block: the degrees below the first tile (left untouched on request).
-  if (lt0 > 0 && !skip_zero) {
-    if constexpr (VOUT) {
-      const int n4 = lt0 / 4;
-      for (int idx = tid; idx < kRowsPerBlock * n4; idx += 256) {
-        const int q = idx / n4, c = idx - q * n4;
-        const int row = row0 + q;
-        if (row >= nrows) continue;
-        int b, r;
-        row_field(m, row, &b, &r);
-        *reinterpret_cast<float4*>(modal + ((int64_t)b * Rrows + r) * L + 4 * c) =
-            make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-    } else {
-      // Rows of any alignment (odd L): per row, float4 stores from the first
-      // 16-byte boundary on, and single floats for the up to 3 degrees
-      // before it and the up to 3 after the last whole float4.  Work items
-      // per row: n4max float4 slots, then 4 head and 4 tail slots.
-      const int n4max = lt0 / 4, per = n4max + 8;
-      for (int idx = tid; idx < kRowsPerBlock * per; idx += 256) {
-        const int q = idx / per, c = idx - q * per;
-        const int row = row0 + q;
-        if (row >= nrows) continue;
-        int b, r;
-        row_field(m, row, &b, &r);
-        float* const p = modal + ((int64_t)b * Rrows + r) * L;
-        const int head = (int)((4 - ((reinterpret_cast<uintptr_t>(p) >> 2) & 3)) & 3);
-        const int h = head < lt0 ? head : lt0;
-        const int n4 = (lt0 - h) / 4;
-        if (c < n4max) {
-          if (c < n4) *reinterpret_cast<float4*>(p + h + 4 * c) = make_float4(0.f, 0.f, 0.f, 0.f);
-        } else {
-          const int j = c - n4max;
-          if (j < 4) {
-            if (j < h) p[j] = 0.f;
-          } else {
-            const int t = h + 4 * n4 + (j - 4);
-            if (t < lt0) p[t] = 0.f;
-          }
-        }
-      }
-    }
-  }
-
-  // Output rows of this lane as offsets from the block's first field (the
-  // block spans at most 64 fields, so they fit in 32 bits), -1 when out of
-  // range.  Direct stores: rows 4 g + i of the wave's strip; staged stores:
-  // rows 2 rr + lane / 32.  Out-of-range lanes store into `sink` instead of
-  // branching around the store, so every store instruction issues and the
-  // next tile's wait for its table loads can leave the stores in flight.
+  // Output.  Every row is written in whole 128-byte lines aligned in memory
+  // (float4 stores): a TL row holds an odd number of degrees, so tile-sized
+  // segments would start anywhere in a line and split into partial writes.
+  // A per-row LDS ring of 64 degrees (two tiles) keeps the degrees of the
+  // previous tile that a line straddling the tile boundary still needs;
+  // degree l of a row sits at ring slot (ra + l) & 63, ra = row address / 4
+  // mod 64, so a line is contiguous in the ring.  With a = (ra + lt0) & 31,
+  // tile lt writes the line starting at degree lt - a (and the next one on
+  // the last tile); quads with degrees outside [lo, L) -- the previous row's
+  // tail, the next row's head, the zero block in triangle-only mode -- are
+  // written float by float.  A row's last line also takes the next row's
+  // zero head (next_row_head).  Out-of-range lanes store into `sink`
+  // instead of branching around the store, so every store
+  // instruction issues and the next tile's wait for its table loads can
+  // leave the stores in flight.
   float* const obase = modal + (int64_t)(m == 0 ? row0 : row0 >> 1) * Rrows * L;
-  constexpr int NOR = VOUT ? 4 : 8;
-  int orow[NOR];
-#pragma unroll
-  for (int i = 0; i < NOR; ++i) {
-    const int rl = VOUT ? 4 * g + i : 2 * i + (lane >> 5);
+  const int lo = skip_zero ? lt0 : 0;
+  auto row_offset = [&](int rl) -> int {  // from obase (at most 64 fields: 32 bits), -1 if none
     const int row = row0 + wave * 16 + rl;
-    orow[i] = -1;
-    if (row < nrows) {
+    if (row >= nrows) return -1;
+    int b, r;
+    row_field(m, row, &b, &r);
+    return ((b - (m == 0 ? row0 : row0 >> 1)) * Rrows + r) * L;
+  };
+  auto ring_origin = [&](int off) -> int {
+    return off < 0 ? 0 : (int)((reinterpret_cast<uintptr_t>(obase + off) >> 2) & 63);
+  };
+  // rows this lane stores: quad store j -> row 8 j + lane / 8, quad lane % 8;
+  // hi = end of what the row's writer stores (L plus the next row's head)
+  int orow[2], ra_s[2], hi[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    orow[j] = row_offset(8 * j + (lane >> 3));
+    ra_s[j] = ring_origin(orow[j]);
+    hi[j] = L;
+    if (orow[j] >= 0 && !skip_zero) {
       int b, r;
-      row_field(m, row, &b, &r);
-      orow[i] = ((b - (m == 0 ? row0 : row0 >> 1)) * Rrows + r) * L;
+      row_field(m, row0 + wave * 16 + 8 * j + (lane >> 3), &b, &r);
+      hi[j] = L + next_row_head(obase + orow[j] + L, r, Rrows, L);
     }
   }
+  // rows this lane fills in the ring: 4 g + i
+  int ra_w[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) ra_w[i] = ring_origin(row_offset(4 * g + i));
+  float* const rg = ring + wave * 16 * kRingPitch;
+  // the ring starts as zeros: the degrees just below lt0 that the first
+  // tile's first line covers are zero-block degrees
+  for (int i = lane; i < 16 * kRingPitch / 4; i += 64) reinterpret_cast<floatx4*>(rg)[i] = zero;
 
   for (int tt = 0; tt < ntile; ++tt) {
     floatx4* buf = lds + (tt & 1) * TF4;
@@ -310,31 +347,38 @@ __global__ __launch_bounds__(256) void leg_fwd_rows_kernel(
       }
     }
     const int lt = lt0 + tt * kTileDeg;
-    if constexpr (VOUT) {
-      const int l = lt + 2 * c16;
+    // this tile's degrees lt + 2 c16 + q of rows 4 g + i into the ring
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float* dst = (orow[i] >= 0 && l < L) ? obase + orow[i] + l : sink + 2 * lane;
-        *reinterpret_cast<float2*>(dst) = make_float2(a0[i], a1[i]);
-      }
-    } else {
-      // Unaligned rows: transpose the wave's 16 x 32 tile through LDS so that
-      // each store instruction writes two contiguous 128-byte row segments.
-      float* os = ostage + wave * (16 * kOutPitch);
+    for (int i = 0; i < 4; ++i) {
+      float* const rr = rg + (4 * g + i) * kRingPitch;
+      rr[(ra_w[i] + lt + 2 * c16) & 63] = a0[i];
+      rr[(ra_w[i] + lt + 2 * c16 + 1) & 63] = a1[i];
+    }
+    wave_lds_sync();  // the ring is private to the wave
+    const int nk = tt + 1 < ntile ? 1 : 2;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        os[(4 * g + i) * kOutPitch + 2 * c16] = a0[i];
-        os[(4 * g + i) * kOutPitch + 2 * c16 + 1] = a1[i];
-      }
-      wave_lds_sync();  // the staging tile is private to the wave
-      const int col = lane & 31, l = lt + col;
+    for (int jk = 0; jk < 4; ++jk) {
+      const int j = jk & 1, k = jk >> 1;
+      if (k >= nk) break;
+      const int rl = 8 * j + (lane >> 3);
+      const int l4 = lt - ((ra_s[j] + lt0) & 31) + 32 * k + 4 * (lane & 7);
+      const floatx4 v =
+          *reinterpret_cast<const floatx4*>(rg + rl * kRingPitch + ((ra_s[j] + l4) & 63));
+      floatx4 w;  // degrees >= L: the next row's zero head
 #pragma unroll
-      for (int rr = 0; rr < 8; ++rr) {
-        const int rl = 2 * rr + (lane >> 5);
-        float* dst = (orow[rr] >= 0 && l < L) ? obase + orow[rr] + l : sink + lane;
-        *dst = os[rl * kOutPitch + col];
+      for (int e = 0; e < 4; ++e) w[e] = l4 + e < L ? v[e] : 0.f;
+      const bool rok = orow[j] >= 0;
+      const bool full = rok && l4 >= lo && l4 + 3 < hi[j];
+      floatx4* const dst = full ? reinterpret_cast<floatx4*>(obase + orow[j] + l4)
+                                : reinterpret_cast<floatx4*>(sink) + lane;
+      *dst = w;
+      if (rok && !full && l4 + 3 >= lo && l4 < hi[j]) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (l4 + e >= lo && l4 + e < hi[j]) obase[orow[j] + l4 + e] = w[e];
       }
     }
+    wave_lds_sync();  // reads of this tile's lines before the next tile's ring writes
   }
 }
 
@@ -386,19 +430,16 @@ template <bool BF, int KS>
 int launch_rows(const dino_plan* pl, const float* in, float* out, int64_t B, int skip_zero,
                 hipStream_t s) {
   const bool vin = pl->J % 8 == 0 && reinterpret_cast<uintptr_t>(in) % 16 == 0;
-  const bool vout = pl->L % 4 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0;
   const dim3 grid((unsigned)((2 * B + kRowsPerBlock - 1) / kRowsPerBlock), (unsigned)pl->M);
   const float4* tab = BF ? reinterpret_cast<const float4*>(pl->ffragbf)
                          : reinterpret_cast<const float4*>(pl->ffrag32);
   const int mpair = (pl->sched[DINO_STAGE_LEGENDRE_FORWARD] & DINO_SCHED_MPAIRED) ? 1 : 0;
-#define ROWS_LAUNCH(VI, VO)                                                                  \
-  leg_fwd_rows_kernel<BF, KS, VI, VO><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, pl->sink, \
-                                                            (int)B, pl->L, pl->J, pl->Jh, pl->R,  \
-                                                            mpair, skip_zero)
-  if (vin && vout) ROWS_LAUNCH(true, true);
-  else if (vin) ROWS_LAUNCH(true, false);
-  else if (vout) ROWS_LAUNCH(false, true);
-  else ROWS_LAUNCH(false, false);
+#define ROWS_LAUNCH(VI)                                                                   \
+  leg_fwd_rows_kernel<BF, KS, VI><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, pl->sink, \
+                                                        (int)B, pl->L, pl->J, pl->Jh, pl->R,  \
+                                                        mpair, skip_zero)
+  if (vin) ROWS_LAUNCH(true);
+  else ROWS_LAUNCH(false);
 #undef ROWS_LAUNCH
   DINO_LAUNCH_CHECK("legendre forward rows kernel");
   return DINO_OK;

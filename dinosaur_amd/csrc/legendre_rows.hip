@@ -260,15 +260,16 @@ __global__ __launch_bounds__(256, KS <= 4 ? 3 : 1) void leg_fwd_rows_kernel(
   // degree l of a row sits at ring slot (ra + l) & 63, ra = row address / 4
   // mod 64, so a line is contiguous in the ring.  With a = (ra + lt0) & 31,
   // tile lt writes the line starting at degree lt - a (and the next one on
-  // the last tile); quads with degrees outside [lo, L) -- the previous row's
-  // tail, the next row's head, the zero block in triangle-only mode -- are
-  // written float by float.  A row's last line also takes the next row's
-  // zero head (next_row_head).  Out-of-range lanes store into `sink`
-  // instead of branching around the store, so every store
+  // the last tile).  The degrees lt0 - a .. lt0 - 1 of the first line are
+  // zeros from the ring, also in triangle-only mode (entries below lt0 may be
+  // zeroed there), and a row's last line also takes the next row's zero head
+  // (next_row_head), so lines are whole except where a row's first line
+  // begins in the previous row's non-zero tail: quads with degrees outside
+  // [0, hi) are written float by float.  Out-of-range lanes store into
+  // `sink` instead of branching around the store, so every store
   // instruction issues and the next tile's wait for its table loads can
   // leave the stores in flight.
   float* const obase = modal + (int64_t)(m == 0 ? row0 : row0 >> 1) * Rrows * L;
-  const int lo = skip_zero ? lt0 : 0;
   auto row_offset = [&](int rl) -> int {  // from obase (at most 64 fields: 32 bits), -1 if none
     const int row = row0 + wave * 16 + rl;
     if (row >= nrows) return -1;
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(256, KS <= 4 ? 3 : 1) void leg_fwd_rows_kernel(
     orow[j] = row_offset(8 * j + (lane >> 3));
     ra_s[j] = ring_origin(orow[j]);
     hi[j] = L;
-    if (orow[j] >= 0 && !skip_zero) {
+    if (orow[j] >= 0) {
       int b, r;
       row_field(m, row0 + wave * 16 + 8 * j + (lane >> 3), &b, &r);
       hi[j] = L + next_row_head(obase + orow[j] + L, r, Rrows, L);
@@ -368,14 +369,14 @@ __global__ __launch_bounds__(256, KS <= 4 ? 3 : 1) void leg_fwd_rows_kernel(
 #pragma unroll
       for (int e = 0; e < 4; ++e) w[e] = l4 + e < L ? v[e] : 0.f;
       const bool rok = orow[j] >= 0;
-      const bool full = rok && l4 >= lo && l4 + 3 < hi[j];
+      const bool full = rok && l4 >= 0 && l4 + 3 < hi[j];
       floatx4* const dst = full ? reinterpret_cast<floatx4*>(obase + orow[j] + l4)
                                 : reinterpret_cast<floatx4*>(sink) + lane;
       *dst = w;
-      if (rok && !full && l4 + 3 >= lo && l4 < hi[j]) {
+      if (rok && !full && l4 + 3 >= 0 && l4 < hi[j]) {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-          if (l4 + e >= lo && l4 + e < hi[j]) obase[orow[j] + l4 + e] = w[e];
+          if (l4 + e >= 0 && l4 + e < hi[j]) obase[orow[j] + l4 + e] = w[e];
       }
     }
     wave_lds_sync();  // reads of this tile's lines before the next tile's ring writes

@@ -48,8 +48,11 @@ enum {
  * and dino_legendre_forward: each modal row of zonal wavenumber m is written
  * only from degree s(m) = max(0, L - 32 * ceil((L - m) / 32)) on (the first
  * degree of the 32-degree tiles that cover m .. L-1, aligned to end at L),
- * and the entries below (all zero, since P_l^m vanishes for l < m) are left
- * untouched.  For callers whose consumers never read them -- the inverse
+ * and each entry below (all zero, since P_l^m vanishes for l < m) either
+ * keeps what the buffer held or is set to zero: the rows are stored in
+ * whole 128-byte lines, so the line that holds degree s(m) and the line a
+ * row shares with the start of the next row of the field may zero a few of
+ * them (at most 31 per row side).  For callers whose consumers never read them -- the inverse
  * Legendre kernels start at s(m), and the fused explicit-terms kernels treat
  * l < m as zero -- it saves writing about 45% of a TL255 modal array.  No
  * reference counterpart (XLA writes the dense einsum result). */

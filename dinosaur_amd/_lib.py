@@ -20,7 +20,7 @@ DINO_F32, DINO_F64 = 0, 1
 PREC_NATIVE, PREC_BF16X3 = 0, 2
 # transform stages and launch schedules (dino_plan_set_schedule)
 STAGE_LEGENDRE_INVERSE, STAGE_LEGENDRE_FORWARD, STAGE_DFT_INVERSE, STAGE_DFT_FORWARD = 0, 1, 2, 3
-SCHED_AUTO, SCHED_ROWS, SCHED_TILED, SCHED_TILED_NARROW = 0, 1, 2, 3
+SCHED_AUTO, SCHED_ROWS, SCHED_TILED, SCHED_TILED_NARROW, SCHED_ROWS_WIDE = 0, 1, 2, 3, 4
 SCHED_PERSISTENT, SCHED_PER_TILE = 1, 2
 SCHED_MPAIRED = 0x100
 TARGET_ACCUMULATE, TARGET_IDENTITY, TARGET_RANGED = 1, 2, 4

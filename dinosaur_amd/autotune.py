@@ -33,6 +33,8 @@ from . import _lib
 LEGENDRE_CANDIDATES = (
     ('rows', _lib.SCHED_ROWS),
     ('rows_mpaired', _lib.SCHED_ROWS | _lib.SCHED_MPAIRED),
+    ('rows_wide', _lib.SCHED_ROWS_WIDE),
+    ('rows_wide_mpaired', _lib.SCHED_ROWS_WIDE | _lib.SCHED_MPAIRED),
     ('tiled', _lib.SCHED_TILED),
     ('tiled_mpaired', _lib.SCHED_TILED | _lib.SCHED_MPAIRED),
     ('tiled_narrow', _lib.SCHED_TILED_NARROW),

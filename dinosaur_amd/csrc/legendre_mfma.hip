@@ -348,7 +348,7 @@ int inv_by_width(const dino_plan* pl, const float* in, float* out, int64_t B, bo
 }  // namespace
 
 int legendre_forward_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
-                          bool split, bool triangle_only, hipStream_t s);
+                          bool split, bool triangle_only, bool wide, hipStream_t s);
 int legendre_inverse_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
                           bool split, hipStream_t s);
 
@@ -378,9 +378,11 @@ int legendre_forward_mfma(const dino_plan* pl, const void* fourier, void* modal,
   const float* in = static_cast<const float*>(fourier);
   float* out = static_cast<float*>(modal);
   const int fam = pl->sched[DINO_STAGE_LEGENDRE_FORWARD] & 0xff;
-  if (pl->fks > 0 && (fam == DINO_SCHED_AUTO || fam == DINO_SCHED_ROWS) &&
+  if (pl->fks > 0 &&
+      (fam == DINO_SCHED_AUTO || fam == DINO_SCHED_ROWS || fam == DINO_SCHED_ROWS_WIDE) &&
       (precision == DINO_PREC_NATIVE || precision == DINO_PREC_BF16X3))
-    return legendre_forward_rows(pl, in, out, B, precision == DINO_PREC_BF16X3, triangle_only, s);
+    return legendre_forward_rows(pl, in, out, B, precision == DINO_PREC_BF16X3, triangle_only,
+                                 fam == DINO_SCHED_ROWS_WIDE, s);
   DINO_REQUIRE((2 * B + TR - 1) / TR <= 65535, DINO_EINVAL, "batch too large");
   switch (precision) {
     case DINO_PREC_NATIVE:

@@ -29,7 +29,7 @@ namespace {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kRowsPerBlock = 64;
+constexpr int kRowsPerWave = 16;
 constexpr int kTileDeg = 32;
 constexpr int kRingPitch = 68;  // output ring row pitch (floats): 64 degrees + 4
 static_assert(kRingPitch % 4 == 0 && kRingPitch >= 64, "float4-aligned rows of 64 degrees");
@@ -122,28 +122,40 @@ __device__ __forceinline__ void zero_block(float* __restrict__ modal, int m, int
 
 // VIN: J % 8 == 0 and a 16-byte aligned Fourier buffer (coalesced float4 row
 // loads staged through LDS), otherwise a scalar gather for any J.  The modal
-// rows may have any length and alignment (TL grids have odd L).  Up to 128
-// northern nodes (KS <= 4) the registers allow 3 waves per SIMD, which the
-// LDS budget (3 workgroups per CU) matches.
-template <bool BF, int KS, bool VIN>
-__global__ __launch_bounds__(256, KS <= 4 ? 3 : 1) void leg_fwd_rows_kernel(
+// rows may have any length and alignment (TL grids have odd L).
+// NW waves (16 rows each) per workgroup.  Every workgroup streams all P tiles
+// of its m from L2, so 8 waves (128 rows) halve that traffic per row
+// against 4 -- these requests queue with the HBM reads for the same L1
+// miss slots.  Up to 128 northern nodes (KS <= 4) the registers allow 3
+// waves per SIMD with 4-wave workgroups (3 per CU, as the LDS budget does)
+// and 4 per SIMD with 8-wave ones (2 per CU).
+template <int NW, int KS>
+constexpr int fwd_waves_per_simd() {
+  return KS > 4 ? 1 : (NW == 8 ? 4 : 3);
+}
+
+template <bool BF, int KS, bool VIN, int NW>
+__global__ __launch_bounds__(64 * NW, (fwd_waves_per_simd<NW, KS>())) void leg_fwd_rows_kernel(
     const float* __restrict__ fourier, float* __restrict__ modal,
     const float4* __restrict__ ftab, const int64_t* __restrict__ ftile_off,
     float* __restrict__ sink, int B, int L, int J, int Jh, int Rrows, int mpair, int skip_zero) {
   constexpr int KT = 2 * KS;  // 16-node groups
+  constexpr int NTH = 64 * NW;  // threads
+  constexpr int RPB = kRowsPerWave * NW;  // rows per workgroup
   constexpr int TF4 = tile_f4(KS);
-  constexpr int PER = TF4 / 256;
-  // One LDS region: first the staging of the folded rows (64 rows, pitch
-  // padded by 4 floats against bank conflicts), then the table double buffer.
+  constexpr int PER = (TF4 + NTH - 1) / NTH;
+  // One LDS region: first the staging of the folded rows (RPB rows, pitch
+  // padded by 4 floats against bank conflicts), then the table double buffer
+  // followed by the output rings (one per wave).
   constexpr int PITCH = 32 * KS + 4;
-  constexpr int STG4 = kRowsPerBlock * PITCH / 4;
-  constexpr int LDS4 = (2 * TF4 > STG4) ? 2 * TF4 : STG4;
+  constexpr int STG4 = RPB * PITCH / 4;
+  constexpr int RING4 = NW * kRowsPerWave * kRingPitch / 4;
+  constexpr int LDS4 = (2 * TF4 + RING4 > STG4) ? 2 * TF4 + RING4 : STG4;
   __shared__ floatx4 lds[LDS4];
-  __shared__ floatx4 ring4[4 * 16 * kRingPitch / 4];  // float4 aligned
-  float* const ring = reinterpret_cast<float*>(ring4);
+  float* const ring = reinterpret_cast<float*>(lds + 2 * TF4);
   const int m = m_of_block(blockIdx.y, gridDim.y, mpair);
   const int nrows = (m == 0) ? B : 2 * B;
-  const int row0 = blockIdx.x * kRowsPerBlock;
+  const int row0 = blockIdx.x * RPB;
   if (row0 >= nrows) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, c16 = lane & 15;
@@ -155,7 +167,10 @@ __global__ __launch_bounds__(256, KS <= 4 ? 3 : 1) void leg_fwd_rows_kernel(
   // the staging array in VGPRs so the loads stay asynchronous).
   floatx4 st[PER];
 #pragma unroll
-  for (int i = 0; i < PER; ++i) st[i] = ld4(tb + tid + 256 * i);
+  for (int i = 0; i < PER; ++i) {
+    const int idx = tid + NTH * i;
+    st[i] = ld4(tb + (idx < TF4 ? idx : 0));
+  }
 
   // Folded rows: X0 pairs with the even-d degrees of a tile (l - m of
   // parity (lt0 - m) & 1), X1 with the odd-d ones.  When lt0 - m is odd the
@@ -208,7 +223,8 @@ __global__ __launch_bounds__(256, KS <= 4 ? 3 : 1) void leg_fwd_rows_kernel(
         if (pass == 0) x0[t] = make_float4(w[0], w[1], w[2], w[3]);
         else x1[t] = make_float4(w[0], w[1], w[2], w[3]);
       }
-      // pass 1: the whole block reuses the staging LDS as table buffers next
+      // pass 1: the whole block reuses the staging LDS as table buffers and
+      // rings next
       if (pass == 0) wave_lds_sync();
       else __syncthreads();
     }
@@ -270,7 +286,7 @@ __global__ __launch_bounds__(256, KS <= 4 ? 3 : 1) void leg_fwd_rows_kernel(
   // instruction issues and the next tile's wait for its table loads can
   // leave the stores in flight.
   float* const obase = modal + (int64_t)(m == 0 ? row0 : row0 >> 1) * Rrows * L;
-  auto row_offset = [&](int rl) -> int {  // from obase (at most 64 fields: 32 bits), -1 if none
+  auto row_offset = [&](int rl) -> int {  // from obase (at most 128 fields: 32 bits), -1 if none
     const int row = row0 + wave * 16 + rl;
     if (row >= nrows) return -1;
     int b, r;
@@ -306,13 +322,19 @@ __global__ __launch_bounds__(256, KS <= 4 ? 3 : 1) void leg_fwd_rows_kernel(
   for (int tt = 0; tt < ntile; ++tt) {
     floatx4* buf = lds + (tt & 1) * TF4;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) buf[tid + 256 * i] = st[i];
+    for (int i = 0; i < PER; ++i) {
+      const int idx = tid + NTH * i;
+      if (idx < TF4) buf[idx] = st[i];
+    }
     __syncthreads();
     {
       // the last iteration reloads its own tile, which is never stored
       const float4* nb = tb + (int64_t)(tt + 1 < ntile ? tt + 1 : tt) * TF4;
 #pragma unroll
-      for (int i = 0; i < PER; ++i) st[i] = ld4(nb + tid + 256 * i);
+      for (int i = 0; i < PER; ++i) {
+        const int idx = tid + NTH * i;
+        st[i] = ld4(nb + (idx < TF4 ? idx : 0));
+      }
     }
     // keep the prefetch ahead of the MFMAs (the scheduler would otherwise
     // sink the loads to the end of the tile, serialising them with it)
@@ -427,18 +449,19 @@ __global__ void forward_fragments_kernel(int L, int Jh, int Jhp, int ks,
   }
 }
 
-template <bool BF, int KS>
+template <bool BF, int KS, int NW>
 int launch_rows(const dino_plan* pl, const float* in, float* out, int64_t B, int skip_zero,
                 hipStream_t s) {
+  constexpr int RPB = kRowsPerWave * NW;
   const bool vin = pl->J % 8 == 0 && reinterpret_cast<uintptr_t>(in) % 16 == 0;
-  const dim3 grid((unsigned)((2 * B + kRowsPerBlock - 1) / kRowsPerBlock), (unsigned)pl->M);
+  const dim3 grid((unsigned)((2 * B + RPB - 1) / RPB), (unsigned)pl->M);
   const float4* tab = BF ? reinterpret_cast<const float4*>(pl->ffragbf)
                          : reinterpret_cast<const float4*>(pl->ffrag32);
   const int mpair = (pl->sched[DINO_STAGE_LEGENDRE_FORWARD] & DINO_SCHED_MPAIRED) ? 1 : 0;
 #define ROWS_LAUNCH(VI)                                                                   \
-  leg_fwd_rows_kernel<BF, KS, VI><<<grid, 256, 0, s>>>(in, out, tab, pl->ftile_off, pl->sink, \
-                                                        (int)B, pl->L, pl->J, pl->Jh, pl->R,  \
-                                                        mpair, skip_zero)
+  leg_fwd_rows_kernel<BF, KS, VI, NW><<<grid, 64 * NW, 0, s>>>(                            \
+      in, out, tab, pl->ftile_off, pl->sink, (int)B, pl->L, pl->J, pl->Jh, pl->R, mpair,  \
+      skip_zero)
   if (vin) ROWS_LAUNCH(true);
   else ROWS_LAUNCH(false);
 #undef ROWS_LAUNCH
@@ -446,14 +469,14 @@ int launch_rows(const dino_plan* pl, const float* in, float* out, int64_t B, int
   return DINO_OK;
 }
 
-template <bool BF>
+template <bool BF, int NW>
 int rows_by_ks(const dino_plan* pl, const float* in, float* out, int64_t B, int skip_zero,
                hipStream_t s) {
   switch (pl->fks) {
-    case 1: return launch_rows<BF, 1>(pl, in, out, B, skip_zero, s);
-    case 2: return launch_rows<BF, 2>(pl, in, out, B, skip_zero, s);
-    case 4: return launch_rows<BF, 4>(pl, in, out, B, skip_zero, s);
-    case 8: return launch_rows<BF, 8>(pl, in, out, B, skip_zero, s);
+    case 1: return launch_rows<BF, 1, NW>(pl, in, out, B, skip_zero, s);
+    case 2: return launch_rows<BF, 2, NW>(pl, in, out, B, skip_zero, s);
+    case 4: return launch_rows<BF, 4, NW>(pl, in, out, B, skip_zero, s);
+    case 8: return launch_rows<BF, 8, NW>(pl, in, out, B, skip_zero, s);
     default:
       set_error("forward fragment tables hold %d steps", pl->fks);
       return DINO_EINVAL;
@@ -491,14 +514,18 @@ void forward_fragment_fill(const dino_plan* pl, hipStream_t s) {
 bool forward_rows_available(const dino_plan* pl) { return pl->fks > 0; }
 
 int legendre_forward_rows(const dino_plan* pl, const float* in, float* out, int64_t B,
-                          bool split, bool triangle_only, hipStream_t s) {
+                          bool split, bool triangle_only, bool wide, hipStream_t s) {
   if (B == 0) return DINO_OK;
-  DINO_REQUIRE(2 * B / kRowsPerBlock < (int64_t)1 << 30, DINO_EINVAL, "batch too large");
-  // a block's output rows span at most 64 fields, addressed with 32-bit offsets
-  DINO_REQUIRE((int64_t)kRowsPerBlock * pl->R * pl->L < (int64_t)1 << 31, DINO_EINVAL,
+  DINO_REQUIRE(2 * B / kRowsPerWave < (int64_t)1 << 30, DINO_EINVAL, "batch too large");
+  // a block's output rows span at most 128 fields, addressed with 32-bit offsets
+  DINO_REQUIRE((int64_t)8 * kRowsPerWave * pl->R * pl->L < (int64_t)1 << 31, DINO_EINVAL,
                "modal fields too large for the row kernel");
   const int z = triangle_only ? 1 : 0;
-  return split ? rows_by_ks<true>(pl, in, out, B, z, s) : rows_by_ks<false>(pl, in, out, B, z, s);
+  if (wide)
+    return split ? rows_by_ks<true, 8>(pl, in, out, B, z, s)
+                 : rows_by_ks<false, 8>(pl, in, out, B, z, s);
+  return split ? rows_by_ks<true, 4>(pl, in, out, B, z, s)
+               : rows_by_ks<false, 4>(pl, in, out, B, z, s);
 }
 
 }  // namespace dino

@@ -312,9 +312,11 @@ extern "C" int dino_plan_set_schedule(dino_plan* p, int stage, int schedule) {
                    (legendre || flags == 0),
                DINO_EINVAL, "dino_plan_set_schedule: invalid flags in schedule 0x%x", schedule);
   if (legendre) {
-    DINO_REQUIRE(fam <= DINO_SCHED_TILED_NARROW, DINO_EINVAL,
+    DINO_REQUIRE(fam <= DINO_SCHED_ROWS_WIDE, DINO_EINVAL,
                  "dino_plan_set_schedule: invalid Legendre kernel family %d", fam);
-    if (fam == DINO_SCHED_ROWS) {
+    DINO_REQUIRE(fam != DINO_SCHED_ROWS_WIDE || stage == DINO_STAGE_LEGENDRE_FORWARD,
+                 DINO_EINVAL, "dino_plan_set_schedule: the wide row kernel is forward only");
+    if (fam == DINO_SCHED_ROWS || fam == DINO_SCHED_ROWS_WIDE) {
       const bool have = stage == DINO_STAGE_LEGENDRE_INVERSE ? p->int_nt > 0 : p->fks > 0;
       DINO_REQUIRE(have, DINO_EINVAL,
                    "dino_plan_set_schedule: no row-resident Legendre kernel for %d latitudes",

@@ -114,6 +114,8 @@ enum {
   DINO_SCHED_ROWS = 1,         /* Legendre: row-resident kernel, all degree tiles of one m per workgroup */
   DINO_SCHED_TILED = 2,        /* Legendre: (rows x nodes/degrees) tiled kernel, wide tiles */
   DINO_SCHED_TILED_NARROW = 3, /* Legendre: tiled kernel, narrow tiles (more workgroups) */
+  DINO_SCHED_ROWS_WIDE = 4,    /* forward Legendre: row-resident kernel with 128-row
+                                  workgroups (half the P-table traffic per row) */
   DINO_SCHED_PERSISTENT = 1,   /* DFT: persistent FP32 FFT workgroups (one wave per CU) */
   DINO_SCHED_PER_TILE = 2,     /* DFT: one workgroup per (field, latitude tile) */
   DINO_SCHED_MPAIRED = 0x100,  /* Legendre flag: m order 0, M-1, 1, M-2, ... (long and

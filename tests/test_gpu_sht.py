@@ -234,3 +234,45 @@ def test_triangle_only_flag_leaves_zero_block_untouched(fn, gpu_device):
     below += low.numel()
     assert torch.equal(out[:, r, lt0:], full[:, r, lt0:]), r
   assert kept >= 0.5 * below, (kept, below)
+
+
+@pytest.mark.parametrize('name', ['T21', 'TL127', 'TL255', 'j36_l16'])
+def test_wide_forward_rows_bitwise_equal(name, gpu_device):
+  """The forward row kernel with 128-row workgroups (8 waves) does the same
+  arithmetic per row as the 64-row one: outputs are bitwise equal, with
+  batches that leave the last workgroup partly empty (and half the rows at
+  m = 0), full and triangle-only writes, FP32 and split-BF16x3, and the
+  scalar-load path (J % 8 != 0, j36_l16)."""
+  import ctypes
+  from dinosaur_amd import _lib
+  grids = {'T21': sh.Grid.T21(), 'TL127': sh.Grid.TL127(), 'TL255': sh.Grid.TL255(),
+           'j36_l16': sh.Grid(16, 16, 48, 36)}
+  plan = grids[name].spherical_harmonics.plan(gpu_device)
+  lib = _lib.lib()
+  stream = ctypes.c_void_p(torch.cuda.current_stream(gpu_device).cuda_stream)
+  p = lambda t: ctypes.c_void_p(t.data_ptr())
+  st = _lib.STAGE_LEGENDRE_FORWARD
+  old = lib.dino_plan_get_schedule(plan.handle, st)
+  batches = (3, 70) if name == 'TL255' else (1, 33, 70, 130)
+  try:
+    for b in batches:
+      rs = np.random.RandomState(b)
+      four = torch.tensor(rs.normal(size=(b, plan.R, plan.J)), dtype=torch.float32,
+                          device=gpu_device)
+      for prec in (_lib.PREC_NATIVE, _lib.PREC_BF16X3):
+        for flag in (0, _lib.SHT_TRIANGLE_ONLY):
+          outs = []
+          for sched in (_lib.SCHED_ROWS, _lib.SCHED_ROWS_WIDE,
+                        _lib.SCHED_ROWS_WIDE | _lib.SCHED_MPAIRED):
+            _lib.check(lib.dino_plan_set_schedule(plan.handle, st, sched), 'schedule')
+            out = torch.full((b, plan.R, plan.L), float('nan'), device=gpu_device)
+            _lib.check(lib.dino_legendre_forward(plan.handle, p(four), p(out), b, _lib.DINO_F32,
+                                                 prec | flag, stream), 'forward')
+            torch.cuda.synchronize()
+            outs.append(out.nan_to_num(nan=7.0))
+          print(f'[test] wide rows {name} B={b} prec={prec} flag={flag}', flush=True)
+          assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
+          if not flag:
+            assert not torch.isnan(outs[0]).any()
+  finally:
+    lib.dino_plan_set_schedule(plan.handle, st, old)

@@ -52,8 +52,9 @@ enum {
  * keeps what the buffer held or is set to zero: the rows are stored in
  * whole 128-byte lines, so the line that holds degree s(m) and the line a
  * row shares with the start of the next row of the field may zero a few of
- * them (at most 31 per row side).  For callers whose consumers never read them -- the inverse
- * Legendre kernels start at s(m), and the fused explicit-terms kernels treat
+ * them (at most 31 per row side).  For callers whose consumers never read
+ * them -- the inverse Legendre kernels start at s(m), and the fused
+ * explicit-terms kernels treat
  * l < m as zero -- it saves writing about 45% of a TL255 modal array.  No
  * reference counterpart (XLA writes the dense einsum result). */
 enum { DINO_SHT_TRIANGLE_ONLY = 0x100 };
@@ -75,8 +76,8 @@ int dino_gauss_nodes_host(int n, double* nodes, double* weights);
 
 /* Builds a transform plan: FP64 Gauss nodes/weights (or the given nodes and
  * weights) and the FP64 normalised P_l^m table generated on the GPU, folded
- * onto the northern hemisphere, plus FP32 / split-FP16 / split-BF16 copies
- * and the longitude twiddle table.  nodes/weights: host arrays of length J,
+ * onto the northern hemisphere, plus FP32 / split-BF16 copies and the
+ * longitude twiddle table.  nodes/weights: host arrays of length J,
  * ascending and symmetric about 0, or NULL for Gauss nodes.
  * Replaces RealSphericalHarmonics.basis
  * (dinosaur/spherical_harmonic.py:219-260) together with

@@ -9,10 +9,10 @@
 //   * a workgroup owns 64 rows (4 waves x 16) of one zonal wavenumber m and
 //     keeps the accumulators of every node tile in VGPRs while it sweeps the
 //     32-degree tiles of that m;
-//   * per tile, each wave loads its 16 row segments with coalesced loads
-//     (32 consecutive degrees of a row per half-wave; the rows of TL grids
-//     have an odd number of degrees, so the loads are dword loads) and turns
-//     them into A-operand fragments through a small LDS buffer;
+//   * per tile, each row of the workgroup brings in one whole 128-byte line
+//     of its modal row (float4 loads; TL rows hold an odd number of degrees,
+//     so rows start anywhere in a line) into a two-line LDS ring per row,
+//     from which the waves read their A-operand fragments;
 //   * the Legendre values arrive through a double-buffered LDS tile filled
 //     from a plan table stored in fragment order (inverse_fragments_kernel);
 //   * at the end the up / down halves are staged in LDS and written as whole
@@ -33,7 +33,7 @@ typedef short shortx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kRows = 64;
 constexpr int kTile = 32;
-constexpr int kAPitch = 36;  // A staging row pitch (floats), 16-byte aligned rows
+constexpr int kRingPitch = 68;  // modal ring row pitch (floats): 64 degrees + 4
 
 __device__ __forceinline__ void row_field(int m, int q, int* b, int* r) {
   if (m == 0) {
@@ -74,17 +74,22 @@ __global__ __launch_bounds__(256 * WS, WS == 2 && !BF ? 6 : 1) void leg_inv_rows
   static_assert(NT % WS == 0, "the waves of a strip split the node tiles evenly");
   constexpr int NTH = 256 * WS;  // threads
   constexpr int NTW = NT / WS;   // node tiles per wave
-  constexpr int RPW = 8 / WS;    // modal values per lane per tile
+  constexpr int RPL = 2 / WS;    // modal line quads per lane per tile
   constexpr int TF4 = inv_tile_f4(NT);
   constexpr int PER = (TF4 + NTH - 1) / NTH;
   constexpr int OPITCH = 16 * NT + 4;
   constexpr int OUT4 = 4 * 16 * OPITCH / 4;
   constexpr int MAIN4 = (2 * TF4 > OUT4) ? 2 * TF4 : OUT4;
-  // A staging: one buffer per strip, double-buffered when two waves share it
-  // (one may write the next tile while the other still reads this one)
-  constexpr int NAS = WS == 1 ? 1 : 2;
   __shared__ floatx4 lds[MAIN4];
-  __shared__ float astg[NAS * 4 * 16 * kAPitch];
+  // Modal rows arrive as whole 128-byte lines (float4 loads, one new line
+  // per row and tile) into a per-row ring of two lines: degree l of a row
+  // sits at slot (ra + l) & 63, ra = row address / 4 mod 64, so a line is
+  // contiguous there, and the tile lt .. lt + 31 reads the two lines that
+  // hold it.  A TL row holds an odd number of degrees, so tile-sized
+  // segments would start anywhere in a line and split into partial
+  // requests.
+  __shared__ floatx4 ring4[4 * 16 * kRingPitch / 4];
+  float* const ring = reinterpret_cast<float*>(ring4);
   const int m = m_of_block(blockIdx.y, gridDim.y, mpair);
   const int nrows = (m == 0) ? B : 2 * B;
   const int row0 = blockIdx.x * kRows;
@@ -95,32 +100,71 @@ __global__ __launch_bounds__(256 * WS, WS == 2 && !BF ? 6 : 1) void leg_inv_rows
   const int lt0 = leg_tile_start(m, L);
   const int ntile = inv_tiles_of_m(m, L);
   const float4* tb = itab + itile_off[m];
+  float* const rs = ring + strip * 16 * kRingPitch;  // this strip's rings
 
-  // prefetch registers: table tile and this wave's rows of the strip's 16 x 32
-  // modal segment (rows 2 rr + lane / 32, rr = r WS + wh)
-  floatx4 st[PER];
-  float ar[RPW];
-  auto load_tile = [&](int tt) {
+  // Line loads of this lane: quad q of row rl of the strip (item = 8 rl + q;
+  // the strip's 128 items split over its WS waves).  Line k of a row starts
+  // at degree lt0 - a + 32 k, a = (ra + lt0) & 31: line 0 holds lt0.  Rows
+  // past the batch load the last row, and lines past the row's last degree
+  // the row's last line: every load is a whole line that holds an element
+  // of the modal array (no address outside its pages), and what such a line
+  // brings in is zeroed where it is read.
+  const float* const fbase = modal + (int64_t)(m == 0 ? row0 : row0 >> 1) * Rrows * L;
+  int loff[RPL], lmax[RPL], lslot[RPL];
+#pragma unroll
+  for (int i = 0; i < RPL; ++i) {
+    const int item = WS == 2 ? 64 * wh + lane : lane + 64 * i;
+    const int rl = item >> 3, q = item & 7;
+    const int row = row0 + strip * 16 + rl;
+    int b, r;
+    row_field(m, row < nrows ? row : nrows - 1, &b, &r);
+    const int off = ((b - (m == 0 ? row0 : row0 >> 1)) * Rrows + r) * L;  // < 2^31 (checked)
+    const int ra = (int)((reinterpret_cast<uintptr_t>(fbase + off) >> 2) & 63);
+    const int a = (ra + lt0) & 31;
+    loff[i] = off + lt0 - a + 4 * q;         // quad q of line 0
+    lmax[i] = (L - 1 - (lt0 - a)) >> 5;      // last line holding a degree < L
+    lslot[i] = (((ra + lt0) & 32) + 4 * q) | (rl << 8);  // ring slot of quad q, line 0; row
+  }
+  auto load_line = [&](floatx4* dst, int k) {
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+      const int kk = k < lmax[i] ? k : lmax[i];
+      dst[i] = *reinterpret_cast<const floatx4*>(fbase + loff[i] + 32 * kk);
+    }
+  };
+  auto store_line = [&](const floatx4* src, int k) {
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+      const int rl = lslot[i] >> 8;
+      *reinterpret_cast<floatx4*>(rs + rl * kRingPitch + (((lslot[i] & 255) + 32 * k) & 63)) =
+          src[i];
+    }
+  };
+  // reader side: row c16 of the strip
+  int ra_c;
+  bool rok_c;
+  {
+    const int row = row0 + strip * 16 + c16;
+    rok_c = row < nrows;
+    int b, r;
+    row_field(m, rok_c ? row : nrows - 1, &b, &r);
+    const int off = ((b - (m == 0 ? row0 : row0 >> 1)) * Rrows + r) * L;
+    ra_c = (int)((reinterpret_cast<uintptr_t>(fbase + off) >> 2) & 63);
+  }
+
+  // prefetch registers: table tile, and the strip's next modal line
+  floatx4 st[PER], ln0[RPL], ln1[RPL];
+  auto load_table = [&](int tt) {
     const float4* src = tb + (int64_t)tt * TF4;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int idx = tid + NTH * i;
       st[i] = *reinterpret_cast<const floatx4*>(src + (idx < TF4 ? idx : 0));
     }
-    // Raw loads from clamped addresses; invalid entries are zeroed where the
-    // values are consumed, so no instruction here waits for the data and the
-    // loads stay in flight during the next tile's MFMAs.
-    const int l = lt0 + kTile * tt + (lane & 31);
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-      const int row = row0 + strip * 16 + 2 * (r * WS + wh) + (lane >> 5);
-      const bool ok = row < nrows && l < L;
-      int b = 0, rr = 0;
-      if (ok) row_field(m, row, &b, &rr);
-      ar[r] = modal[((int64_t)b * Rrows + rr) * L + (ok ? l : 0)];
-    }
   };
-  load_tile(0);
+  load_table(0);
+  load_line(ln0, 0);
+  load_line(ln1, 1);
 
   floatx4 acc[NTW][2];
 #pragma unroll
@@ -128,30 +172,33 @@ __global__ __launch_bounds__(256 * WS, WS == 2 && !BF ? 6 : 1) void leg_inv_rows
 
   for (int tt = 0; tt < ntile; ++tt) {
     floatx4* buf = lds + (tt & 1) * TF4;
-    float* as = astg + ((NAS == 2 ? (tt & 1) : 0) * 4 + strip) * 16 * kAPitch;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int idx = tid + NTH * i;
       if (idx < TF4) buf[idx] = st[i];
     }
-    {
-      const bool lok = lt0 + kTile * tt + (lane & 31) < L;
-#pragma unroll
-      for (int r = 0; r < RPW; ++r) {
-        const int rl = 2 * (r * WS + wh) + (lane >> 5);
-        const bool ok = lok && row0 + strip * 16 + rl < nrows;
-        as[rl * kAPitch + (lane & 31)] = ok ? ar[r] : 0.f;
-      }
-    }
+    // the ring holds lines tt and tt + 1 from here on
+    if (tt == 0) store_line(ln0, 0);
+    store_line(ln1, tt + 1);
     __syncthreads();
     // the last iteration reloads its own tile, which is never stored
-    load_tile(tt + 1 < ntile ? tt + 1 : tt);
+    load_table(tt + 1 < ntile ? tt + 1 : tt);
+    load_line(ln1, tt + 2);
     // keep the prefetch ahead of the MFMAs (the scheduler would otherwise
     // sink the loads to the end of the tile, serialising them with it)
     __builtin_amdgcn_sched_barrier(0);
-    const floatx4 v0 = *reinterpret_cast<const floatx4*>(as + c16 * kAPitch + 8 * g);
-    const floatx4 v1 = *reinterpret_cast<const floatx4*>(as + c16 * kAPitch + 8 * g + 4);
-    const float v[8] = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
+    const int lt = lt0 + kTile * tt;
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int l = lt + 8 * g + e;
+      const float x = rs[c16 * kRingPitch + ((ra_c + l) & 63)];
+      v[e] = (rok_c && l < L) ? x : 0.f;
+    }
+    // both waves of a strip have read line tt before either writes line
+    // tt + 2 over it (next iteration)
+    if constexpr (WS == 2) __syncthreads();
+    else wave_lds_sync();
     if constexpr (BF) {
       shortx4 hi[2], lo[2];
 #pragma unroll
@@ -342,6 +389,9 @@ int legendre_inverse_rows(const dino_plan* pl, const float* in, float* out, int6
                           bool split, hipStream_t s) {
   if (B == 0) return DINO_OK;
   DINO_REQUIRE(2 * B / kRows < (int64_t)1 << 30, DINO_EINVAL, "batch too large");
+  // a block's modal rows span at most 64 fields, addressed with 32-bit offsets
+  DINO_REQUIRE((int64_t)kRows * pl->R * pl->L < (int64_t)1 << 31, DINO_EINVAL,
+               "modal fields too large for the row kernel");
   return split ? inv_rows_by_nt<true>(pl, in, out, B, s) : inv_rows_by_nt<false>(pl, in, out, B, s);
 }
 

@@ -219,6 +219,37 @@ def lib() -> ctypes.CDLL:
   return _lib
 
 
+# Library objects whose destruction waits for the end of a graph capture.
+_released = []
+
+
+def _capturing() -> bool:
+  try:
+    import torch  # pylint: disable=g-import-not-at-top
+    return torch.cuda.is_available() and torch.cuda.is_current_stream_capturing()
+  except Exception:  # pylint: disable=broad-except  (interpreter shutdown)
+    return False
+
+
+def release(destroy: str, handle) -> None:
+  """Destroys a library object (`destroy` names its dino_*_destroy), or
+  defers that while the current stream is being captured into a HIP graph.
+  The destroy calls free device memory, which a capture does not permit:
+  the capture would be invalidated and the error would surface at the next
+  launch check.  Python's garbage collector may drop an object anywhere,
+  also inside a capture, so every destructor goes through here; deferred
+  objects are destroyed at the next release outside a capture."""
+  if _lib is None:
+    return
+  if _capturing():
+    _released.append((destroy, handle))
+    return
+  while _released:
+    name, h = _released.pop()
+    getattr(_lib, name)(h)
+  getattr(_lib, destroy)(handle)
+
+
 def check(rc: int, what: str = '') -> None:
   if rc == DINO_OK:
     return

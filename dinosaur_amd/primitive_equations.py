@@ -362,7 +362,7 @@ class _ImplicitOperator:
       if _lib._lib is None:
         return
       for h in getattr(self, '_factors', {}).values():
-        _lib._lib.dino_implicit_destroy(h)
+        _lib.release('dino_implicit_destroy', h)
     except Exception:  # pylint: disable=broad-except  (interpreter shutdown)
       pass
 
@@ -451,8 +451,8 @@ class _NodalHandle:
 
   def __del__(self):
     try:
-      if _lib._lib is not None and self.handle.value:
-        _lib._lib.dino_nodal_destroy(self.handle)
+      if self.handle.value:
+        _lib.release('dino_nodal_destroy', self.handle)
     except Exception:  # pylint: disable=broad-except  (interpreter shutdown)
       pass
 
@@ -507,8 +507,8 @@ class _Handle:
 
   def __del__(self):
     try:
-      if _lib._lib is not None and self.handle.value:
-        getattr(_lib._lib, self._destroy)(self.handle)
+      if self.handle.value:
+        _lib.release(self._destroy, self.handle)
     except Exception:  # pylint: disable=broad-except  (interpreter shutdown)
       pass
 

@@ -119,8 +119,8 @@ class Plan:
   def __del__(self):
     try:
       h = getattr(self, 'handle', None)
-      if h is not None and h.value and _lib._lib is not None:
-        _lib._lib.dino_plan_destroy(h)
+      if h is not None and h.value:
+        _lib.release('dino_plan_destroy', h)
     except Exception:  # pylint: disable=broad-except  (interpreter shutdown)
       pass
 

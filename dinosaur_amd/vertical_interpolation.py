@@ -54,8 +54,8 @@ class _Regridder:
 
   def __del__(self):
     try:
-      if _lib._lib is not None and self.handle.value:
-        _lib._lib.dino_vregrid_destroy(self.handle)
+      if self.handle.value:
+        _lib.release('dino_vregrid_destroy', self.handle)
     except Exception:  # pylint: disable=broad-except  (interpreter shutdown)
       pass
 

@@ -89,3 +89,32 @@ def test_work_tally_roofs():
   assert abs(r['leg'] - 2e-3) < 1e-12          # 16 GB = 2 ms, 2 ms of MFMA
   assert abs(r['copy'] - 2e-3) < 1e-12
   assert abs(r['solve'] - 1e-4) < 1e-12        # 0.1 ms of memory, 0.1 ms of FP64
+
+
+def test_release_defers_destroy_during_graph_capture(monkeypatch):
+  """Library objects dropped while a stream is being captured (the garbage
+  collector may run there) are destroyed after the capture, not inside it:
+  their destroy calls free device memory, which would invalidate the
+  capture.  A fake library records the calls."""
+  from dinosaur_amd import _lib
+
+  calls = []
+
+  class Fake:
+    def __getattr__(self, name):
+      return lambda h: calls.append((name, h))
+
+  monkeypatch.setattr(_lib, '_lib', Fake())
+  monkeypatch.setattr(_lib, '_released', [])
+  capturing = [True]
+  monkeypatch.setattr(_lib, '_capturing', lambda: capturing[0])
+  _lib.release('dino_plan_destroy', 1)
+  _lib.release('dino_modal_destroy', 2)
+  assert calls == [] and len(_lib._released) == 2
+  capturing[0] = False
+  _lib.release('dino_nodal_destroy', 3)
+  assert sorted(calls[:2]) == [('dino_modal_destroy', 2), ('dino_plan_destroy', 1)]
+  assert calls[2] == ('dino_nodal_destroy', 3) and _lib._released == []
+  monkeypatch.setattr(_lib, '_lib', None)
+  _lib.release('dino_plan_destroy', 4)  # no library loaded: nothing to do
+  assert len(calls) == 3

@@ -36,6 +36,9 @@ namespace {
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWavesPerBlock = 8;  // waves sharing one LDS copy of the tables
+// 16-column tiles each wave processes: the block's copy of the tables (tens
+// of KiB from L2) then serves kRepeats x 8 tiles instead of 8
+constexpr int kRepeats = 4;
 
 __device__ __forceinline__ doublex4 mma64(double a, double b, doublex4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -95,7 +98,7 @@ __device__ __forceinline__ void zero_columns(T* div_out, T* temp_out, T* lnps_ou
 // In place is allowed (outputs equal to inputs): a wave reads every level of
 // its 16 columns before it writes any, and no other wave touches them.
 template <typename T, int NS, bool TRANS>
-__global__ __launch_bounds__(64 * kWavesPerBlock, (!TRANS && sizeof(T) == 4 && NS <= 10) ? 4 : 1)
+__global__ __launch_bounds__(64 * kWavesPerBlock, (sizeof(T) == 4 && NS <= 10) ? 4 : 1)
 void solve_mfma_kernel(
     const T* div, const T* temp, const T* lnps, T* div_out, T* temp_out, T* lnps_out,
     const double* __restrict__ t1a_g, const double* __restrict__ t1b_g,
@@ -115,10 +118,9 @@ void solve_mfma_kernel(
   const int T2 = stage2_tiles(K);
   for (int i = threadIdx.x; i < T2 * NS * 64; i += blockDim.x) t2[i] = t2_g[i];
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tiles_per_member = (cols + 15) / 16;
-  if (wave >= tiles_per_member * members) return;
+  // one 16-column tile (a return ends this tile only)
+  auto tile = [&](int64_t wave, int lane) {
   const int64_t b = wave / tiles_per_member;
   const int64_t c = (wave - b * tiles_per_member) * 16 + (lane & 15);
   const int kg = lane >> 4;
@@ -197,6 +199,18 @@ void solve_mfma_kernel(
   }
   // the lnps row lives in one k-group
   if (kg == (lrow & 3) && cok) lnps_out[b * cols + cc] = (T)(lp - c2 * sacc);
+  };
+  const int64_t first = (int64_t)blockIdx.x * kRepeats * kWavesPerBlock + (threadIdx.x >> 6);
+#pragma unroll 1
+  for (int rep = 0; rep < kRepeats; ++rep) {
+    const int64_t wave = first + (int64_t)rep * kWavesPerBlock;
+    if (wave >= tiles_per_member * members) break;
+    // an opaque copy of the lane index keeps the table reads inside the loop
+    // (hoisted out of it they would hold ~220 registers)
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    tile(wave, lane);
+  }
 }
 
 // TRANS = false: the implicit terms.  TRANS = true: their transpose
@@ -218,10 +232,9 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void terms_mfma_kernel(
   for (int i = threadIdx.x; i < NG; i += blockDim.x) tg[i] = tg_g[i];
   for (int i = threadIdx.x; i < NH; i += blockDim.x) th[i] = th_g[i];
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t tiles_per_member = (cols + 15) / 16;
-  if (wave >= tiles_per_member * members) return;
+  // one 16-column tile (a return ends this tile only)
+  auto tile = [&](int64_t wave, int lane) {
   const int64_t b = wave / tiles_per_member;
   const int64_t c = (wave - b * tiles_per_member) * 16 + (lane & 15);
   const int kg = lane >> 4;
@@ -289,6 +302,18 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void terms_mfma_kernel(
     T* o = lnps_out + b * cols + cc;
     const T v = (T)(-s2 * sacc);
     *o = ACC ? (T)(*o + v) : v;
+  }
+  };
+  const int64_t first = (int64_t)blockIdx.x * kRepeats * kWavesPerBlock + (threadIdx.x >> 6);
+#pragma unroll 1
+  for (int rep = 0; rep < kRepeats; ++rep) {
+    const int64_t wave = first + (int64_t)rep * kWavesPerBlock;
+    if (wave >= tiles_per_member * members) break;
+    // an opaque copy of the lane index keeps the table reads inside the loop
+    // (hoisted out of it they would hold ~220 registers)
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    tile(wave, lane);
   }
 }
 
@@ -391,7 +416,8 @@ static int launch_ns(bool solve, bool adjoint, bool acc, const dino_implicit* f,
   const int64_t cols = (int64_t)f->R * f->L;
   const int64_t waves = (cols + 15) / 16 * members;
   if (waves == 0) return DINO_OK;
-  const unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+  const int64_t per_block = (int64_t)kWavesPerBlock * kRepeats;
+  const unsigned blocks = (unsigned)((waves + per_block - 1) / per_block);
   const unsigned threads = 64 * kWavesPerBlock;
   const T* x0 = (const T*)div;
   const T* x1 = (const T*)temp;

@@ -98,6 +98,22 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Workgroups of a launch are dispatched to the GPU's 8 XCDs, each with its
+// own L2, in turn: hardware workgroup n (x fastest) runs on XCD n mod 8.
+// xcd_block maps it to a logical workgroup (bx, by) such that each XCD takes
+// one contiguous eighth of the logical grid, so workgroups that read each
+// other's neighbours (stencils across rows) share an L2.  A permutation of
+// the grid: correct whatever the dispatch order is.
+__device__ __forceinline__ void xcd_block(int* bx, int* by) {
+  const int gx = (int)gridDim.x;
+  const int64_t n = (int64_t)gx * gridDim.y;
+  const int64_t lin = blockIdx.x + (int64_t)blockIdx.y * gx;
+  const int64_t q = n / 8, r = n % 8, k = lin % 8, i = lin / 8;
+  const int64_t logical = (k < r ? k * (q + 1) : r * (q + 1) + (k - r) * q) + i;
+  *bx = (int)(logical % gx);
+  *by = (int)(logical / gx);
+}
+
 // Zonal wavenumber handled by grid row y of a Legendre launch over M rows:
 // natural order (longest triangular slab first) or paired (0, M-1, 1, M-2,
 // ...), which alternates long and short slabs.

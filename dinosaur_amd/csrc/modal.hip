@@ -120,7 +120,7 @@ inline EpiGroups epi_groups(int K, int nt, bool moist, int64_t E) {
 struct MP {
   int R, L, K, nt, hum;  // hum < 0 also on hybrid levels (no humidity extras)
   int hybrid;
-  int kc, nkc;  // levels per thread and level chunks: blockIdx.y = member * nkc + chunk
+  int kc, nkc;  // levels per thread and level chunks: (logical) grid row = member * nkc + chunk
   int64_t RL;
   double a, g;
 };
@@ -206,9 +206,11 @@ __global__ __launch_bounds__(256) void prep_kernel(const T* __restrict__ st, T* 
                                                    const int* __restrict__ tile_pos,
                                                    int64_t n_pos, StateGroups sg, PrepGroups pg,
                                                    MP p) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t e = blockIdx.y / p.nkc;
-  const int kch = blockIdx.y % p.nkc;
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int64_t q = (int64_t)bx * blockDim.x + threadIdx.x;
+  const int64_t e = by / p.nkc;
+  const int kch = by % p.nkc;
   if (q >= n_pos) return;
   const int64_t pos = tile_pos[q];
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);
@@ -262,9 +264,11 @@ __global__ __launch_bounds__(256) void prep_adjoint_kernel(const T* __restrict__
                                                            T* __restrict__ gst,
                                                            const T* __restrict__ cst,
                                                            StateGroups sg, PrepGroups pg, MP p) {
-  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t e = blockIdx.y / p.nkc;
-  const int kch = blockIdx.y % p.nkc;
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int64_t pos = (int64_t)bx * blockDim.x + threadIdx.x;
+  const int64_t e = by / p.nkc;
+  const int kch = by % p.nkc;
   if (pos >= p.RL) return;
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);
   const MOff o = moff(p.R, p.L);
@@ -313,9 +317,11 @@ __global__ __launch_bounds__(256) void epilogue_kernel(const T* __restrict__ in,
                                                        T* __restrict__ st,
                                                        const T* __restrict__ cst, EpiGroups eg,
                                                        StateGroups sg, MP p) {
-  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t e = blockIdx.y / p.nkc;
-  const int kch = blockIdx.y % p.nkc;
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int64_t pos = (int64_t)bx * blockDim.x + threadIdx.x;
+  const int64_t e = by / p.nkc;
+  const int kch = by % p.nkc;
   if (pos >= p.RL) return;
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);
   const MOff o = moff(p.R, p.L);
@@ -369,9 +375,11 @@ __global__ __launch_bounds__(256) void epilogue_adjoint_kernel(const T* __restri
                                                                const int* __restrict__ tile_pos,
                                                                int64_t n_pos, EpiGroups eg,
                                                                StateGroups sg, MP p) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t e = blockIdx.y / p.nkc;
-  const int kch = blockIdx.y % p.nkc;
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int64_t q = (int64_t)bx * blockDim.x + threadIdx.x;
+  const int64_t e = by / p.nkc;
+  const int kch = by % p.nkc;
   if (q >= n_pos) return;
   const int64_t pos = tile_pos[q];
   const int r = (int)(pos / p.L), l = (int)(pos % p.L);

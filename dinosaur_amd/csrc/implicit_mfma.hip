@@ -75,13 +75,15 @@ __device__ __forceinline__ double ldd(const T* p) {
 // member is (row r, degree l) with zonal wavenumber m = (r + 1) / 2, and
 // fields vanish for l < m.  A wave whose 16 columns all lie there has zero
 // input and zero output, so it reads nothing.
-__device__ __forceinline__ bool below_triangle(int tri, int64_t c, int64_t cols, int L) {
-  bool lane_tri = true;
-  if (c < cols) {
-    const int r = (int)(c / L), l = (int)(c - (int64_t)r * L);
-    lane_tri = l < ((r + 1) >> 1);
-  }
+__device__ __forceinline__ bool below_triangle(int tri, bool cok, int r, int l) {
+  const bool lane_tri = cok ? l < ((r + 1) >> 1) : true;
   return tri && __all(lane_tri);
+}
+
+// Column c (< cols = R L, so 32-bit) of a member: row r and degree l.
+__device__ __forceinline__ void column_rl(int c, int L, int* r, int* l) {
+  *r = c / L;
+  *l = c - *r * L;
 }
 
 template <typename T>
@@ -110,31 +112,38 @@ void solve_mfma_kernel(
   // the block's waves share the operand tables through LDS (one L2 read per
   // block instead of one per wave and MFMA)
   constexpr int N1 = S::T1 * NS * 64, N2 = S::T2MAX * NS * 64;
-  __shared__ double t1a[N1], t1b[N1], t2[N2];
+  // (and the per-row vectors of the z scaling, read per tile)
+  __shared__ double t1a[N1], t1b[N1], t2[N2], wv[16 * S::T1], dv[16 * S::T1];
   for (int i = threadIdx.x; i < N1; i += blockDim.x) {
     t1a[i] = t1a_g[i];
     t1b[i] = t1b_g[i];
   }
   const int T2 = stage2_tiles(K);
   for (int i = threadIdx.x; i < T2 * NS * 64; i += blockDim.x) t2[i] = t2_g[i];
+  for (int i = threadIdx.x; i < 16 * S::T1; i += blockDim.x) {
+    wv[i] = wvec[i];
+    dv[i] = dvec[i];
+  }
   __syncthreads();
-  const int64_t tiles_per_member = (cols + 15) / 16;
+  const int tiles_per_member = (int)((cols + 15) / 16);
   // one 16-column tile (a return ends this tile only)
-  auto tile = [&](int64_t wave, int lane) {
-  const int64_t b = wave / tiles_per_member;
-  const int64_t c = (wave - b * tiles_per_member) * 16 + (lane & 15);
+  auto tile = [&](int wave, int lane) {
+  const int b = wave / tiles_per_member;
+  const int c = (wave - b * tiles_per_member) * 16 + (lane & 15);
   const int kg = lane >> 4;
   const bool cok = c < cols;
-  const int64_t cc = cok ? c : 0;
-  const int64_t base = b * (int64_t)K * cols + cc;
-  if (below_triangle(tri, c, cols, L)) {
+  const int cc = cok ? c : 0;
+  int r, l;
+  column_rl(cc, L, &r, &l);
+  const int64_t base = (int64_t)b * K * cols + cc;
+  if (below_triangle(tri, cok, r, l)) {
     // every column of the wave has l < m: zero in, zero out
-    if (!inplace && !ACCUM) zero_columns<T>(div_out, temp_out, lnps_out, base, b * cols + c, cols,
-                                            K, kg, c < cols);
+    if (!inplace && !ACCUM) zero_columns<T>(div_out, temp_out, lnps_out, base,
+                                            (int64_t)b * cols + c, cols, K, kg, cok);
     return;
   }
-  const double lam_l = lam[(int)(cc % L)];
-  const double lp = ldd(lnps + b * cols + cc);
+  const double lam_l = lam[l];
+  const double lp = ldd(lnps + (int64_t)b * cols + cc);
   // stage-1 B fragments: delta and T rows 4 s + kg (zero beyond K)
   double xd[NS], xt[NS];
 #pragma unroll
@@ -146,6 +155,8 @@ void solve_mfma_kernel(
     xd[s] = ok ? dv : 0.0;
     xt[s] = ok ? tv : 0.0;
   }
+  // all row tiles of stage 1 at once: 2 T1 independent accumulator chains
+  // keep the matrix cores busy (one tile at a time, 2 chains, is slower)
   doublex4 ca[S::T1], cb[S::T1];
 #pragma unroll
   for (int it = 0; it < S::T1; ++it) {
@@ -167,8 +178,8 @@ void solve_mfma_kernel(
       if (4 * it + i >= NS) continue;
       const int row = 16 * it + 4 * i + kg;
       const double c1 = TRANS ? eta : eta * lam_l;
-      const double rhs = ca[it][i] - c1 * (cb[it][i] + wvec[row] * lp);
-      z[4 * it + i] = rhs / (1.0 - eta * eta * lam_l * dvec[row]);
+      const double rhs = ca[it][i] - c1 * (cb[it][i] + wv[row] * lp);
+      z[4 * it + i] = rhs / (1.0 - eta * eta * lam_l * dv[row]);
     }
   }
   // stage 2, one output row tile at a time: rows [0, KP) delta, [KP, KP + K)
@@ -176,12 +187,7 @@ void solve_mfma_kernel(
   const double c2 = TRANS ? eta * lam_l : eta;
   const int lrow = S::KP + K;
   double sacc = 0.0;
-#pragma unroll
-  for (int gt = 0; gt < S::T2MAX; ++gt) {
-    if (gt >= T2) continue;  // uniform: K = KP takes the last tile
-    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int s = 0; s < NS; ++s) acc = mma64(t2[(gt * NS + s) * 64 + lane], z[s], acc);
+  auto emit = [&](int gt, const doublex4& acc) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int row = 16 * gt + 4 * i + kg;
@@ -196,15 +202,31 @@ void solve_mfma_kernel(
         sacc = acc[i];
       }
     }
+  };
+  // output row tiles in pairs: two independent accumulator chains back to
+  // back (all of them at once would hold every table fragment in registers)
+#pragma unroll
+  for (int g0 = 0; g0 < S::T2MAX; g0 += 2) {
+    if (g0 >= T2) continue;  // uniform: K = KP takes the last tile
+    const bool two = g0 + 1 < T2;
+    doublex4 a0 = {0.0, 0.0, 0.0, 0.0}, a1 = a0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      a0 = mma64(t2[(g0 * NS + s) * 64 + lane], z[s], a0);
+      if (two) a1 = mma64(t2[((g0 + 1) * NS + s) * 64 + lane], z[s], a1);
+    }
+    emit(g0, a0);
+    if (two) emit(g0 + 1, a1);
   }
   // the lnps row lives in one k-group
-  if (kg == (lrow & 3) && cok) lnps_out[b * cols + cc] = (T)(lp - c2 * sacc);
+  if (kg == (lrow & 3) && cok) lnps_out[(int64_t)b * cols + cc] = (T)(lp - c2 * sacc);
   };
-  const int64_t first = (int64_t)blockIdx.x * kRepeats * kWavesPerBlock + (threadIdx.x >> 6);
+  const int first = (int)blockIdx.x * kRepeats * kWavesPerBlock + (int)(threadIdx.x >> 6);
+  const int n_waves = tiles_per_member * (int)members;
 #pragma unroll 1
   for (int rep = 0; rep < kRepeats; ++rep) {
-    const int64_t wave = first + (int64_t)rep * kWavesPerBlock;
-    if (wave >= tiles_per_member * members) break;
+    const int wave = first + rep * kWavesPerBlock;
+    if (wave >= n_waves) break;
     // an opaque copy of the lane index keeps the table reads inside the loop
     // (hoisted out of it they would hold ~220 registers)
     int lane = threadIdx.x & 63;
@@ -228,27 +250,30 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void terms_mfma_kernel(
   constexpr bool ACCUM = ACC;  // adding the zero result changes nothing
   using S = Shape<NS>;
   constexpr int NG = S::T1 * NS * 64, NH = S::TT * NS * 64;
-  __shared__ double tg[NG], th[NH];
+  __shared__ double tg[NG], th[NH], rt[16 * S::T1];
   for (int i = threadIdx.x; i < NG; i += blockDim.x) tg[i] = tg_g[i];
   for (int i = threadIdx.x; i < NH; i += blockDim.x) th[i] = th_g[i];
+  for (int i = threadIdx.x; i < 16 * S::T1; i += blockDim.x) rt[i] = rtref[i];
   __syncthreads();
-  const int64_t tiles_per_member = (cols + 15) / 16;
+  const int tiles_per_member = (int)((cols + 15) / 16);
   // one 16-column tile (a return ends this tile only)
-  auto tile = [&](int64_t wave, int lane) {
-  const int64_t b = wave / tiles_per_member;
-  const int64_t c = (wave - b * tiles_per_member) * 16 + (lane & 15);
+  auto tile = [&](int wave, int lane) {
+  const int b = wave / tiles_per_member;
+  const int c = (wave - b * tiles_per_member) * 16 + (lane & 15);
   const int kg = lane >> 4;
   const bool cok = c < cols;
-  const int64_t cc = cok ? c : 0;
-  const int64_t base = b * (int64_t)K * cols + cc;
-  if (below_triangle(tri, c, cols, L)) {
+  const int cc = cok ? c : 0;
+  int r, l;
+  column_rl(cc, L, &r, &l);
+  const int64_t base = (int64_t)b * K * cols + cc;
+  if (below_triangle(tri, cok, r, l)) {
     // every column of the wave has l < m: zero in, zero out
-    if (!inplace && !ACCUM) zero_columns<T>(div_out, temp_out, lnps_out, base, b * cols + c, cols,
-                                            K, kg, c < cols);
+    if (!inplace && !ACCUM) zero_columns<T>(div_out, temp_out, lnps_out, base,
+                                            (int64_t)b * cols + c, cols, K, kg, cok);
     return;
   }
-  const double lam_l = lam[(int)(cc % L)];
-  const double lp = ldd(lnps + b * cols + cc);
+  const double lam_l = lam[l];
+  const double lp = ldd(lnps + (int64_t)b * cols + cc);
   double xd[NS], xt[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
@@ -259,56 +284,64 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void terms_mfma_kernel(
     xd[s] = ok ? dv : 0.0;
     xt[s] = ok ? tv : 0.0;
   }
-  // divergence tendency: -lam (G T + R T_ref lnps)
-#pragma unroll
-  for (int it = 0; it < S::T1; ++it) {
-    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int s = 0; s < NS; ++s) acc = mma64(tg[(it * NS + s) * 64 + lane], xt[s], acc);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = 16 * it + 4 * i + kg;
-      const double s1 = TRANS ? 1.0 : lam_l;
-      if (cok && row < K) {
-        T* o = div_out + base + (int64_t)row * cols;
-        const T v = (T)(-s1 * (acc[i] + rtref[row] * lp));
-        *o = ACC ? (T)(*o + v) : v;
-      }
-    }
-  }
-  // temperature and log-pressure tendencies: -[H; dsigma] delta
+  // divergence tendency -lam (G T + R T_ref lnps) (row tiles it < T1) and
+  // temperature and log-pressure tendencies -[H; dsigma] delta (it < TT):
+  // the two accumulator chains of a row tile index run back to back.  When
+  // accumulating into the outputs (loads and stores of outputs that may
+  // alias), the divergence tiles all come first, then the others (faster
+  // there).
+  const double s1 = TRANS ? 1.0 : lam_l;
   const double s2 = TRANS ? lam_l : 1.0;
   double sacc = 0.0;
+  constexpr int TM = S::T1 > S::TT ? S::T1 : S::TT;
+  constexpr int PH = ACC ? 2 : 1;
 #pragma unroll
-  for (int it = 0; it < S::TT; ++it) {
-    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int ph = 0; ph < PH; ++ph) {
 #pragma unroll
-    for (int s = 0; s < NS; ++s) acc = mma64(th[(it * NS + s) * 64 + lane], xd[s], acc);
+  for (int it = 0; it < TM; ++it) {
+    const bool dg = it < S::T1 && (!ACC || ph == 0);
+    const bool dh = it < S::TT && (!ACC || ph == 1);
+    if (!dg && !dh) continue;
+    doublex4 ag = {0.0, 0.0, 0.0, 0.0}, ah = ag;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if (dg) ag = mma64(tg[(it * NS + s) * 64 + lane], xt[s], ag);
+      if (dh) ah = mma64(th[(it * NS + s) * 64 + lane], xd[s], ah);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int row = 16 * it + 4 * i + kg;
-      if (row < K) {
-        if (cok) {
-          T* o = temp_out + base + (int64_t)row * cols;
-          const T v = (T)(-s2 * acc[i]);
-          *o = ACC ? (T)(*o + v) : v;
+      if (dg && cok && row < K) {
+        T* o = div_out + base + (int64_t)row * cols;
+        const T v = (T)(-s1 * (ag[i] + rt[row] * lp));
+        *o = ACC ? (T)(*o + v) : v;
+      }
+      if (dh) {
+        if (row < K) {
+          if (cok) {
+            T* o = temp_out + base + (int64_t)row * cols;
+            const T v = (T)(-s2 * ah[i]);
+            *o = ACC ? (T)(*o + v) : v;
+          }
+        } else if (row == S::KP) {
+          sacc = ah[i];
         }
-      } else if (row == S::KP) {
-        sacc = acc[i];
       }
     }
   }
+  }
   if (kg == (S::KP & 3) && cok) {
-    T* o = lnps_out + b * cols + cc;
+    T* o = lnps_out + (int64_t)b * cols + cc;
     const T v = (T)(-s2 * sacc);
     *o = ACC ? (T)(*o + v) : v;
   }
   };
-  const int64_t first = (int64_t)blockIdx.x * kRepeats * kWavesPerBlock + (threadIdx.x >> 6);
+  const int first = (int)blockIdx.x * kRepeats * kWavesPerBlock + (int)(threadIdx.x >> 6);
+  const int n_waves = tiles_per_member * (int)members;
 #pragma unroll 1
   for (int rep = 0; rep < kRepeats; ++rep) {
-    const int64_t wave = first + (int64_t)rep * kWavesPerBlock;
-    if (wave >= tiles_per_member * members) break;
+    const int wave = first + rep * kWavesPerBlock;
+    if (wave >= n_waves) break;
     // an opaque copy of the lane index keeps the table reads inside the loop
     // (hoisted out of it they would hold ~220 registers)
     int lane = threadIdx.x & 63;
@@ -416,6 +449,10 @@ static int launch_ns(bool solve, bool adjoint, bool acc, const dino_implicit* f,
   const int64_t cols = (int64_t)f->R * f->L;
   const int64_t waves = (cols + 15) / 16 * members;
   if (waves == 0) return DINO_OK;
+  // column tiles and columns are indexed with 32-bit integers in the kernels
+  DINO_REQUIRE(waves + (int64_t)kWavesPerBlock * kRepeats < ((int64_t)1 << 31) &&
+                   cols < ((int64_t)1 << 31),
+               DINO_EINVAL, "implicit solve: too many columns for one launch");
   const int64_t per_block = (int64_t)kWavesPerBlock * kRepeats;
   const unsigned blocks = (unsigned)((waves + per_block - 1) / per_block);
   const unsigned threads = 64 * kWavesPerBlock;

@@ -67,7 +67,7 @@ def test_autotune_batch_buckets_and_lookup():
 @pytest.mark.parametrize('rows,degrees', [(43, 23), (511, 257), (85, 44), (1, 1)])
 def test_triangle_entries_counts_mask(rows, degrees):
   """Entries with |m| <= l of a Real-layout (2M-1, L) array, m order
-  0, +1, -1, +2, -2, ...; matches the SURVEY cheat-sheet's N_real."""
+  0, +1, -1, +2, -2, ... (TL255, (511, 257): 66,047 entries)."""
   m = np.array([0] + [k for j in range(1, (rows + 1) // 2) for k in (j, j)])
   l = np.arange(degrees)
   assert _lib.triangle_entries(rows, degrees) == int((m[:, None] <= l[None, :]).sum())

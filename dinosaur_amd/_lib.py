@@ -26,6 +26,7 @@ SCHED_MPAIRED = 0x100
 TARGET_ACCUMULATE, TARGET_IDENTITY, TARGET_RANGED = 1, 2, 4
 GATHER_IDENTITY, GATHER_SCALED, GATHER_IMPLICIT_RANGE = 1, 2, 4
 SHT_TRIANGLE_ONLY = 0x100
+STAGE_TRIANGLE_ONLY = 0x100  # dtype flag of dino_imex_stage_ranged[_scaled]
 PRECISIONS = {'native': PREC_NATIVE, 'float32': PREC_NATIVE, 'float64': PREC_NATIVE,
               'bf16x3': PREC_BF16X3}
 

@@ -57,6 +57,7 @@ struct StageArgs {
   int tl;       // ... of tl degrees per row; elements with l < m are zero
   const T* ls;  // non-null: the result is multiplied by ls[i % sl] (a
   int64_t sl;   // per-degree filter applied to the output, one rounding)
+  int keep_tri;  // 1: entries below the triangle are not written
 };
 
 template <typename T>
@@ -80,6 +81,27 @@ __device__ __forceinline__ bool below_triangle(int64_t i, int w, int64_t trl, in
   const int64_t j = i % trl;
   const int r = (int)(j / tl), l = (int)(j - (int64_t)r * tl);
   return l + w - 1 < ((r + 1) >> 1);
+}
+
+// Bit e set: element i + e (e < W) lies below the triangle of its field (the
+// vector may run into the next row, or the next field's first row).
+template <int W>
+__device__ __forceinline__ unsigned below_mask(int64_t i, int64_t trl, int tl) {
+  if (trl <= 0) return 0u;
+  const int64_t j = i % trl;
+  const int R = (int)(trl / tl);
+  int r = (int)(j / tl), l = (int)(j - (int64_t)r * tl);
+  unsigned mask = 0u;
+#pragma unroll
+  for (int e = 0; e < W; ++e) {
+    if (l == tl) {
+      l = 0;
+      r = r + 1 == R ? 0 : r + 1;
+    }
+    if (l < ((r + 1) >> 1)) mask |= 1u << e;
+    ++l;
+  }
+  return mask;
 }
 
 template <typename T>
@@ -127,8 +149,9 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a, T* __restric
   if (k < nv) {
     if constexpr (VEC) {
       const int64_t i0 = k * W;
-      if (below_triangle(i0, W, a.trl, a.tl)) {  // every operand is zero there
-        reinterpret_cast<V*>(out)[k] = V{};
+      const unsigned bm = below_mask<W>(i0, a.trl, a.tl);
+      if (bm == (1u << W) - 1u) {  // every operand is zero there
+        if (!a.keep_tri) reinterpret_cast<V*>(out)[k] = V{};
         return;
       }
       V y = reinterpret_cast<const V*>(a.y0)[k];
@@ -152,13 +175,20 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a, T* __restric
           l = l + 1 == a.sl ? 0 : l + 1;
         }
       }
+      // a vector that the triangle cuts: its entries below it are zero too,
+      // whatever the operands hold there (a stage value may leave them
+      // unwritten, and the implicit terms computed from it then hold
+      // anything there)
+#pragma unroll
+      for (int e = 0; e < W; ++e)
+        if ((bm >> e) & 1u) y[e] = (T)0;
       reinterpret_cast<V*>(out)[k] = y;
-    } else {
+    } else if (!(a.keep_tri && below_triangle(k, 1, a.trl, a.tl))) {
       out[k] = combine_scalar(a, k);
     }
   } else if (VEC && k - nv < n - nv * W) {
     const int64_t i = nv * W + (k - nv);
-    out[i] = combine_scalar(a, i);
+    if (!(a.keep_tri && below_triangle(i, 1, a.trl, a.tl))) out[i] = combine_scalar(a, i);
   }
 }
 
@@ -352,8 +382,9 @@ template <typename T>
 int launch_stage(const void* y0, int n_ex, const void* const* f, const double* a, int n_im,
                  const void* const* g, const double* b, double dt, void* out, int64_t n,
                  int64_t ib, int64_t ie, int64_t go, hipStream_t s, int64_t trl = 0,
-                 int tl = 0, const void* ls = nullptr, int64_t sl = 0) {
+                 int tl = 0, const void* ls = nullptr, int64_t sl = 0, bool keep_tri = false) {
   StageArgs<T> args{};
+  args.keep_tri = keep_tri && trl > 0 ? 1 : 0;
   args.trl = trl;
   args.tl = tl;
   args.ls = static_cast<const T*>(ls);
@@ -540,6 +571,8 @@ int stage_ranged(const char* name, const void* y0, int n_ex, const void* const* 
   DINO_REQUIRE(!scale || (scale_len > 0 && n % scale_len == 0), DINO_EINVAL,
                "%s: the scale length %lld must divide the size %lld", name,
                (long long)scale_len, (long long)n);
+  const bool keep_tri = (dtype & DINO_STAGE_TRIANGLE_ONLY) != 0;
+  dtype &= ~DINO_STAGE_TRIANGLE_ONLY;
   int rc = check_args(name, n_ex, n_im, a, b, n, dtype);
   if (rc != DINO_OK) return rc;
   rc = check_range(name, n, im_begin, im_end, g_origin);
@@ -552,9 +585,11 @@ int stage_ranged(const char* name, const void* y0, int n_ex, const void* const* 
   const int64_t trl = degrees ? field_size : 0;
   if (dtype == DINO_F64)
     return launch_stage<double>(y0, n_ex, f, a, n_im, g, b, dt, out, n, im_begin, im_end,
-                                g_origin, as_stream(stream), trl, degrees, scale, scale_len);
+                                g_origin, as_stream(stream), trl, degrees, scale, scale_len,
+                                keep_tri);
   return launch_stage<float>(y0, n_ex, f, a, n_im, g, b, dt, out, n, im_begin, im_end,
-                             g_origin, as_stream(stream), trl, degrees, scale, scale_len);
+                             g_origin, as_stream(stream), trl, degrees, scale, scale_len,
+                             keep_tri);
 }
 
 }  // namespace

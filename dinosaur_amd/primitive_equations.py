@@ -638,6 +638,13 @@ class _FlatImexOps:
                       spherical_harmonic.RealSphericalHarmonics)
     self.field_size = self.RL if real else 0
     self.degrees = L if real else 0
+    # Stage values may leave the entries below the triangle unwritten when
+    # every reader skips them: the column-local implicit kernels, and the
+    # explicit terms when they run the fused modal kernels (which read
+    # l >= m only); the tensor-op path's derivative recurrences would
+    # multiply those entries by zero coefficients.
+    self.stage_triangle_only = (real and eq.fused_nodal and
+                                type(eq).explicit_terms is PrimitiveEquationsSigma.explicit_terms)
 
   def state(self, buf):
     return State.from_flat(buf, *self.layout)

@@ -227,11 +227,15 @@ def _tally_elementwise(family, ops, like, n_full, n_range):
   _lib.tally(family, like.element_size() * elems * frac)
 
 
-def _fused_stage(ops, y0, dt, ex, im, scale=None):
+def _fused_stage(ops, y0, dt, ex, im, scale=None, triangle_only=False):
   """y0 + dt sum(a f) + dt sum(b g) with compact implicit terms g (one pass),
-  times the per-degree `scale` (a step filter) when given."""
+  times the per-degree `scale` (a step filter) when given.  triangle_only:
+  the entries below the triangle l < m of the Real layout are left
+  uninitialised (a stage value that only the implicit solve and terms,
+  column-local, and the explicit terms, which read l >= m, consume)."""
   from . import _lib  # pylint: disable=g-import-not-at-top
   out = torch.empty_like(y0)
+  code = _code(y0) | (_lib.STAGE_TRIANGLE_ONLY if triangle_only and ops.degrees else 0)
   fs, gs = [x for _, x in ex], [x for _, x in im]
   stream = ctypes.c_void_p(torch.cuda.current_stream(y0.device).cuda_stream)
   args = (ctypes.c_void_p(y0.data_ptr()), len(fs), _ptrs(fs), _dbls([a for a, _ in ex]),
@@ -239,11 +243,11 @@ def _fused_stage(ops, y0, dt, ex, im, scale=None):
           ctypes.c_void_p(out.data_ptr()), y0.numel(), ops.ib, ops.ie, ops.go, ops.field_size,
           ops.degrees)
   if scale is None:
-    rc = _lib.lib().dino_imex_stage_ranged(*args, _code(y0), stream)
+    rc = _lib.lib().dino_imex_stage_ranged(*args, code, stream)
     _lib.check(rc, 'dino_imex_stage_ranged')
   else:
     rc = _lib.lib().dino_imex_stage_ranged_scaled(
-        *args, ctypes.c_void_p(scale.data_ptr()), scale.numel(), _code(y0), stream)
+        *args, ctypes.c_void_p(scale.data_ptr()), scale.numel(), code, stream)
     _lib.check(rc, 'dino_imex_stage_ranged_scaled')
   if _lib._active is not None:
     _tally_elementwise('stage', ops, y0, 2 + len(fs), len(gs))
@@ -378,7 +382,10 @@ def _fused_forward(ops, tab, dt, y0, keep, out_scale=None, need_output=True):
     ex = [(a_ex[i - 1][j], f[j]) for j in range(i) if a_ex[i - 1][j]]
     im = [(a_im[i - 1][j], g[j]) for j in range(i) if a_im[i - 1][j]]
     if need_f[i]:
-      y = _fused_stage(ops, y0, dt, ex, im)
+      # read by the solve (in place), the implicit terms and the explicit
+      # terms only: the entries below the triangle are never read
+      y = _fused_stage(ops, y0, dt, ex, im,
+                       triangle_only=getattr(ops, 'stage_triangle_only', False))
     else:  # only the implicit terms read this stage value
       y = _fused_stage_implicit_range(ops, y0, dt, ex, im)
     ops.solve_(y, dt * a_im[i - 1][i])

@@ -282,9 +282,15 @@ int dino_imex_stage_adjoint(const void* out_cot, int n_ex, const double* a,
  * degrees > 0 declares the buffers whole Real-layout modal fields of
  * field_size = (2M - 1) * degrees elements: entries with l < m (the part of
  * the dense array outside the triangular truncation, zero in every operand
- * of a masked state) may be written as zero without reading any operand --
- * every 16-byte vector of them that lies wholly below the triangle is; the
- * rest are combined as usual.  degrees = 0: plain elementwise. */
+ * of a masked state) are written as zero, whatever the operands hold there
+ * -- a 16-byte vector of them that lies wholly below the triangle without
+ * reading any operand.  degrees = 0: plain elementwise.
+ * DINO_STAGE_TRIANGLE_ONLY OR-ed into dtype (with degrees > 0): the vectors
+ * wholly below the triangle are not written either -- out keeps what it
+ * held there -- for a stage value that only kernels reading the triangle
+ * consume (the implicit solve and terms are column-local, the explicit
+ * terms read l >= m); also accepted by dino_imex_stage_ranged_scaled. */
+enum { DINO_STAGE_TRIANGLE_ONLY = 0x100 };
 int dino_imex_stage_ranged(const void* y0, int n_ex, const void* const* f,
                            const double* a, int n_im, const void* const* g,
                            const double* b, double dt, void* out, int64_t n,

@@ -333,6 +333,63 @@ def test_fused_imex_stage_matches_elementwise_passes(gpu_device):
 
 
 @pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
+def test_stage_triangle_only_flag_leaves_entries_below_the_triangle(dtype, gpu_device):
+  """Entries below the triangle (l < m) of dino_imex_stage_ranged and its
+  scaled form are zero whatever the operands hold there (random here), and
+  with DINO_STAGE_TRIANGLE_ONLY: entries on and above the triangle bitwise as
+  without the flag, entries below it either left as the output buffer held
+  them or, in a 16-byte vector that the triangle cuts, zero; the scalar path
+  (unaligned buffers) leaves every one of them (T21 Real-layout fields)."""
+  import ctypes
+  from dinosaur_amd import _lib
+  print(f'[test] stage triangle-only flag {dtype}', flush=True)
+  lib = _lib.lib()
+  R, L, F = 43, 23, 17
+  fs = R * L
+  n = F * fs
+  ib, ie = 4 * fs, 13 * fs
+  go = ib - ib % 4
+  code = _lib.DINO_F64 if dtype == torch.float64 else _lib.DINO_F32
+  gen = torch.Generator(device=gpu_device).manual_seed(5)
+  stream = ctypes.c_void_p(torch.cuda.current_stream(gpu_device).cuda_stream)
+  P = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])  # noqa: E731
+  D = lambda cs: (ctypes.c_double * len(cs))(*cs)  # noqa: E731
+  r = torch.arange(R).view(R, 1)
+  l = torch.arange(L).view(1, L)
+  on = (l >= (r + 1) // 2).reshape(-1).repeat(F).to(gpu_device)
+  lscale = torch.rand(L, generator=gen, device=gpu_device, dtype=dtype) + 0.5
+  for shift in (0, 1):  # 1: buffers off 16-byte alignment, the scalar path
+    mk = lambda size: torch.randn(size + shift, generator=gen, device=gpu_device,  # noqa: E731
+                                  dtype=dtype)[shift:]
+    y0, f0, f1 = mk(n), mk(n), mk(n)
+    g0 = mk(ie - go)
+    for scaled in (False, True):
+      outs = []
+      for flag in (0, _lib.STAGE_TRIANGLE_ONLY):
+        out = torch.full((n + shift,), float('nan'), device=gpu_device, dtype=dtype)[shift:]
+        args = (ctypes.c_void_p(y0.data_ptr()), 2, P([f0, f1]), D([0.5, -0.25]), 1, P([g0]),
+                D([0.375]), 0.0123, ctypes.c_void_p(out.data_ptr()), n, ib, ie, go, fs, L)
+        if scaled:
+          rc = lib.dino_imex_stage_ranged_scaled(*args, ctypes.c_void_p(lscale.data_ptr()), L,
+                                                 code | flag, stream)
+        else:
+          rc = lib.dino_imex_stage_ranged(*args, code | flag, stream)
+        _lib.check(rc, 'stage')
+        outs.append(out)
+      full, kept = outs
+      assert not torch.isnan(full).any()
+      assert (full[~on] == 0).all(), (shift, scaled)
+      assert torch.equal(kept[on], full[on]), (shift, scaled)
+      below_k, below_f = kept[~on], full[~on]
+      left = torch.isnan(below_k)
+      assert torch.equal(below_k[~left], below_f[~left]), (shift, scaled)
+      if shift:
+        assert left.all(), (scaled, int((~left).sum()))
+      else:
+        assert left.float().mean().item() > 0.5, (scaled, left.float().mean().item())
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
 def test_adjoint_gather_matches_target_passes(dtype, gpu_device):
   """dino_imex_adjoint_gather forms a cotangent from several sources with the
   roundings and order of one dino_imex_stage_adjoint_targets pass writing the

@@ -305,6 +305,7 @@ struct GatherArgs {
   int64_t sl;
   int64_t rb, re;  // the implicit range: DINO_GATHER_IMPLICIT_RANGE sources hold
                    // values there only (zero elsewhere, not read)
+  int keep_tri;    // 1: entries below the triangle are not written
 };
 
 // U = T: idx is the element; U = vector: idx is the vector (elements W idx ..)
@@ -356,14 +357,17 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs<T> a, T* __restr
   auto one = [&](int64_t o) {
     const int64_t i = o + a.off;
     if (i < a.ib || i >= a.ie) return;
-    out[o] = below_triangle(i, 1, a.trl, a.tl) ? (T)0 : gather_sum<T, T>(a, i);
+    const bool tri = below_triangle(i, 1, a.trl, a.tl);
+    if (tri && a.keep_tri) return;
+    out[o] = tri ? (T)0 : gather_sum<T, T>(a, i);
   };
   if (k < nv) {
     if constexpr (VEC) {
       const int64_t o0 = k * W, i0 = o0 + a.off;
       if (i0 >= a.ib && i0 + W <= a.ie) {
-        reinterpret_cast<V*>(out)[k] =
-            below_triangle(i0, W, a.trl, a.tl) ? V{} : gather_sum<T, V>(a, i0 / W);
+        const bool tri = below_triangle(i0, W, a.trl, a.tl);
+        if (!(tri && a.keep_tri))
+          reinterpret_cast<V*>(out)[k] = tri ? V{} : gather_sum<T, V>(a, i0 / W);
       } else {
 #pragma unroll
         for (int e = 0; e < W; ++e) one(o0 + e);
@@ -484,8 +488,9 @@ template <typename T>
 int launch_gather(const void* const* src, const double* coef, const int* flags, int ns,
                   double dt, void* out, int64_t n_out, int64_t off, int64_t ib, int64_t ie,
                   hipStream_t s, int64_t trl, int tl, const void* ls, int64_t sl, int64_t rb,
-                  int64_t re) {
+                  int64_t re, bool keep_tri) {
   GatherArgs<T> args{};
+  args.keep_tri = keep_tri && trl > 0 ? 1 : 0;
   args.ls = static_cast<const T*>(ls);
   args.sl = sl;
   args.rb = rb;
@@ -661,6 +666,8 @@ extern "C" int dino_imex_adjoint_gather(const void* const* src, const double* co
                                         int degrees, const void* degree_scale,
                                         int64_t scale_len, int dtype, void* stream) {
   const char* name = "dino_imex_adjoint_gather";
+  const bool keep_tri = (dtype & DINO_STAGE_TRIANGLE_ONLY) != 0;
+  dtype &= ~DINO_STAGE_TRIANGLE_ONLY;
   DINO_REQUIRE(degrees == 0 || (degrees > 0 && field_size > 0 && field_size % degrees == 0 &&
                                 (field_size / degrees) % 2 == 1 && n % field_size == 0),
                DINO_EINVAL, "%s: invalid Real-layout field shape (%lld, %d)", name,
@@ -699,10 +706,10 @@ extern "C" int dino_imex_adjoint_gather(const void* const* src, const double* co
   if (dtype == DINO_F64)
     return launch_gather<double>(src, coef, flags, n_src, dt, out, n_out, off, ib, ie,
                                  as_stream(stream), trl, degrees, degree_scale, scale_len,
-                                 im_begin, im_end);
+                                 im_begin, im_end, keep_tri);
   return launch_gather<float>(src, coef, flags, n_src, dt, out, n_out, off, ib, ie,
                               as_stream(stream), trl, degrees, degree_scale, scale_len,
-                              im_begin, im_end);
+                              im_begin, im_end, keep_tri);
 }
 
 extern "C" int dino_imex_stage_adjoint(const void* out_cot, int n_ex, const double* a,

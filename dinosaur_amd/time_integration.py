@@ -277,13 +277,15 @@ def _fused_stage_implicit_range(ops, y0, dt, ex, im):
   return out
 
 
-def _fused_gather(ops, sources, dt, like, ranged=False, scale=None):
+def _fused_gather(ops, sources, dt, like, ranged=False, scale=None, triangle_only=False):
   """One cotangent from its pending (source, coef, identity, scaled,
   implicit_only) contributions, summed in list order in one pass (at most 8
   sources: _fused_imex_step takes tableaus of at most 7 stages); scaled
   sources are multiplied by the per-degree `scale` first, implicit_only
-  sources hold values on the implicit range only.  None when there are
-  none."""
+  sources hold values on the implicit range only.  triangle_only: the
+  entries below the triangle are left uninitialised (a cotangent that only
+  the fused explicit terms' adjoint, which reads l >= m, consumes).  None
+  when there are none."""
   from . import _lib  # pylint: disable=g-import-not-at-top
   if not sources:
     return None
@@ -300,7 +302,8 @@ def _fused_gather(ops, sources, dt, like, ranged=False, scale=None):
       len(sources), float(dt), ctypes.c_void_p(out.data_ptr()), like.numel(),
       1 if ranged else 0, ops.ib, ops.ie, ops.go, ops.field_size, ops.degrees,
       ctypes.c_void_p(scale.data_ptr() if scale is not None else 0),
-      scale.numel() if scale is not None else 0, _code(like), stream)
+      scale.numel() if scale is not None else 0,
+      _code(like) | (_lib.STAGE_TRIANGLE_ONLY if triangle_only and ops.degrees else 0), stream)
   _lib.check(rc, 'dino_imex_adjoint_gather')
   if _lib._active is not None:
     n_range = sum(1 for _, _, _, _, r in sources if r) + (1 if ranged else 0)
@@ -442,8 +445,10 @@ class _FusedImexStep(torch.autograd.Function):
     pend_g = {j: [(obar, b_im[j], False, out_sc, False)] for j in range(n)
               if b_im[j] and ctx.has_g[j]}
     pend_y0 = [(obar, 1.0, True, out_sc, False)]
+    # the explicit cotangents feed only the explicit terms' adjoint
+    tri_f = getattr(ops, 'stage_triangle_only', False)
     for i in range(n - 1, 0, -1):
-      fbar = _fused_gather(ops, pend_f.pop(i, None), dt, obar, scale=sc)
+      fbar = _fused_gather(ops, pend_f.pop(i, None), dt, obar, scale=sc, triangle_only=tri_f)
       gbar = _fused_gather(ops, pend_g.pop(i, None), dt, obar, ranged=True, scale=sc)
       ybar = None
       implicit_only = False
@@ -470,7 +475,7 @@ class _FusedImexStep(torch.autograd.Function):
           pend_f.setdefault(j, []).append((ybar, a_ex[i - 1][j], False, False, implicit_only))
         if a_im[i - 1][j] and ctx.has_g[j]:
           pend_g.setdefault(j, []).append((ybar, a_im[i - 1][j], False, False, implicit_only))
-    fbar0 = _fused_gather(ops, pend_f.pop(0, None), dt, obar, scale=sc)
+    fbar0 = _fused_gather(ops, pend_f.pop(0, None), dt, obar, scale=sc, triangle_only=tri_f)
     gbar0 = _fused_gather(ops, pend_g.pop(0, None), dt, obar, ranged=True, scale=sc)
     if f[0] is not None and fbar0 is not None:
       (gy0,) = torch.autograd.grad(f[0], leaves[0], fbar0)

@@ -359,7 +359,9 @@ int dino_imex_stage_adjoint_targets(const void* gbar, int n_targets,
  * only the implicit terms read): it counts as zero elsewhere and is not
  * read there.  1 to 8 sources; out must not overlap them; degree_scale
  * (scale_len elements, scale_len dividing n) may be NULL when no source is
- * scaled. */
+ * scaled.  DINO_STAGE_TRIANGLE_ONLY OR-ed into dtype: the entries that the
+ * kernel would write as zero below the triangle are not written (a
+ * cotangent that only kernels reading l >= m consume). */
 enum { DINO_GATHER_IDENTITY = 1, DINO_GATHER_SCALED = 2, DINO_GATHER_IMPLICIT_RANGE = 4 };
 int dino_imex_adjoint_gather(const void* const* src, const double* coef,
                              const int* flags, int n_src, double dt,

@@ -390,6 +390,60 @@ def test_stage_triangle_only_flag_leaves_entries_below_the_triangle(dtype, gpu_d
 
 
 @pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
+def test_gather_triangle_only_flag_leaves_entries_below_the_triangle(dtype, gpu_device):
+  """DINO_STAGE_TRIANGLE_ONLY on dino_imex_adjoint_gather (full and ranged
+  outputs): every entry the unflagged gather writes with a value is written
+  bitwise the same, the entries it writes as zero below the triangle are
+  either left as the buffer held them or zero, and on the scalar path
+  (unaligned buffers) all of them are left (T21 Real-layout fields)."""
+  import ctypes
+  from dinosaur_amd import _lib
+  print(f'[test] gather triangle-only flag {dtype}', flush=True)
+  lib = _lib.lib()
+  R, L, F = 43, 23, 17
+  fs = R * L
+  n = F * fs
+  ib, ie = 4 * fs, 13 * fs
+  go = ib - ib % 4
+  code = _lib.DINO_F64 if dtype == torch.float64 else _lib.DINO_F32
+  gen = torch.Generator(device=gpu_device).manual_seed(9)
+  stream = ctypes.c_void_p(torch.cuda.current_stream(gpu_device).cuda_stream)
+  P = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])  # noqa: E731
+  D = lambda cs: (ctypes.c_double * len(cs))(*cs)  # noqa: E731
+  I = lambda xs: (ctypes.c_int * len(xs))(*xs)  # noqa: E731
+  r = torch.arange(R).view(R, 1)
+  l = torch.arange(L).view(1, L)
+  below = (l < (r + 1) // 2).reshape(-1).repeat(F).to(gpu_device)
+  for shift in (0, 1):
+    srcs = [torch.randn(n + shift, generator=gen, device=gpu_device, dtype=dtype)[shift:]
+            for _ in range(3)]
+    for ranged in (0, 1):
+      size = ie - go if ranged else n
+      outs = []
+      for flag in (0, _lib.STAGE_TRIANGLE_ONLY):
+        out = torch.full((size + shift,), float('nan'), device=gpu_device, dtype=dtype)[shift:]
+        _lib.check(lib.dino_imex_adjoint_gather(
+            P(srcs), D([0.5, -0.25, 2.0]), I([_lib.GATHER_IDENTITY, 0, 0]), 3, 0.0123,
+            ctypes.c_void_p(out.data_ptr()), n, ranged, ib, ie, go, fs, L, None, 0,
+            code | flag, stream), 'gather')
+        outs.append(out)
+      full, kept = outs
+      bel = below[go:ie] if ranged else below
+      written = ~torch.isnan(full)
+      if ranged:  # [go, ib) lies outside the range: never written
+        assert not written[:ib - go].any()
+      zero_below = written & bel & (full == 0)
+      valued = written & ~(bel & (full == 0))
+      assert torch.equal(kept[valued], full[valued]), (shift, ranged)
+      k = kept[zero_below]
+      assert (torch.isnan(k) | (k == 0)).all(), (shift, ranged)
+      if shift:
+        assert torch.isnan(k).all(), (ranged, int((~torch.isnan(k)).sum()))
+      else:
+        assert torch.isnan(k).float().mean().item() > 0.5, (ranged,)
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
 def test_adjoint_gather_matches_target_passes(dtype, gpu_device):
   """dino_imex_adjoint_gather forms a cotangent from several sources with the
   roundings and order of one dino_imex_stage_adjoint_targets pass writing the
